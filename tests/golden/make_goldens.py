@@ -1,0 +1,100 @@
+"""Writes the golden fixtures of tests/golden/ from the reference nmslib.
+
+Runs only where the reference sources exist: ``make -C oracle ref`` compiles
+them (oracle/Makefile) into oracle/_ref/libnmsref.so; this script drives that
+library through oracle/ref_driver.cc exactly as python_bindings/nmslib.cc
+drives the C++ index (createIndex / saveIndex / loadIndex /
+setQueryTimeParams / knnQueryBatch) and through the brute-force seq_search
+method.  Indices are built single-threaded (indexThreadQty=1) so they are
+reproducible.  Each fixture holds:
+
+  idx_<name>.npz  the index as a pack (nmslib_amd/indexpack.py: everything but
+                  the vectors, which are regenerated from the named seeded
+                  generator) -- unpacking gives the reference's file bytes
+  res_<name>.npz  queries, and the reference's knnQueryBatch ids / distances /
+                  counts for each (efSearch, k, algoType), plus seq_search
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, HERE)
+
+from nmslib_amd import datasets as D  # noqa: E402
+from nmslib_amd import indexpack  # noqa: E402
+from refdrv import RefIndex  # noqa: E402
+
+# name: (space, generator, n, dim, seed, nq, qseed, index params)
+FIXTURES = {
+    "u128_l2": ("l2", "uniform", 2000, 128, 11, 200, 12, "M=16,efConstruction=200"),
+    "cfg1_l2": ("l2", "uniform", 10000, 128, 0, 1000, 1, "M=16,efConstruction=200"),
+    "hn128_l2": ("l2", "halfnormal_u8", 3000, 128, 13, 200, 14, "M=16,efConstruction=500"),
+    "u100_l2": ("l2", "uniform", 1500, 100, 15, 200, 16, "M=8,efConstruction=100"),
+    "dec100_cos": ("cosinesimil", "decaying_normal", 2000, 100, 17, 200, 18,
+                   "M=32,efConstruction=400"),
+    "gmm96_nd": ("negdotprod", "gmm", 2000, 96, 19, 200, 20, "M=24,efConstruction=200"),
+    "tiny_l2": ("l2", "uniform", 60, 128, 21, 50, 22, "M=4,efConstruction=20"),
+}
+
+# (efSearch, k, algoType)
+RUNS = [(10, 10, "hybrid"), (16, 1, "hybrid"), (32, 10, "hybrid"), (100, 10, "hybrid"),
+        (16, 100, "hybrid"), (512, 10, "hybrid"), (800, 100, "hybrid"), (64, 10, "old"),
+        (1000, 10, "hybrid"), (1500, 10, "hybrid"), (100, 10, "v1merge")]
+
+
+def gen(name, n, dim, seed):
+    return getattr(D, name)(n, dim, seed)
+
+
+def main(only=None):
+    meta = {}
+    for name, (space, g, n, dim, seed, nq, qseed, params) in FIXTURES.items():
+        if only and name not in only:
+            continue
+        x = gen(g, n, dim, seed)
+        q = gen(g, nq, dim, qseed)
+        if space == "cosinesimil":
+            q[0] = 0.0  # an all-zero query (bindings_test.py:73-91)
+        ref = RefIndex.build(space, x, params + ",indexThreadQty=1")
+        with tempfile.TemporaryDirectory() as td:
+            path = os.path.join(td, "index.hnsw")
+            ref.save(path)
+            pk = indexpack.pack(path)
+            back = os.path.join(td, "back.hnsw")
+            indexpack.unpack(pk, x, back)
+            with open(path, "rb") as a, open(back, "rb") as b:
+                assert a.read() == b.read(), f"{name}: pack round trip differs"
+            indexpack.save_pack(pk, os.path.join(HERE, f"idx_{name}.npz"))
+            loaded = RefIndex.load(space, path, dim)
+        out = {"queries": q}
+        runs = RUNS if name != "cfg1_l2" else [(100, 10, "hybrid"), (10, 10, "hybrid"),
+                                              (32, 10, "hybrid"), (200, 10, "hybrid")]
+        for ef, k, algo in runs:
+            loaded.set_query_params(f"efSearch={ef},algoType={algo}")
+            ids, dists, counts = loaded.knn_query_batch(q, k, num_threads=1)
+            tag = f"ef{ef}_k{k}_{algo}"
+            out[f"{tag}_ids"], out[f"{tag}_dists"], out[f"{tag}_counts"] = ids, dists, counts
+        kbf = 100 if n >= 100 else n
+        ids, dists, counts = ref.seq_search(q, kbf)
+        out["bf_ids"], out["bf_dists"], out["bf_counts"] = ids, dists, counts
+        np.savez_compressed(os.path.join(HERE, f"res_{name}.npz"), **out)
+        meta[name] = dict(space=space, generator=g, n=n, dim=dim, seed=seed, nq=nq, qseed=qseed,
+                          params=params, runs=[list(r) for r in runs], bf_k=kbf)
+        print(name, "done", flush=True)
+    mpath = os.path.join(HERE, "fixtures.json")
+    old = json.load(open(mpath)) if os.path.exists(mpath) else {}
+    old.update(meta)
+    with open(mpath, "w") as f:
+        json.dump(old, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:] or None)
