@@ -1,0 +1,255 @@
+"""Benchmark of the HNSW query path on one MI355X.
+
+Workload (the 1,000,000 x 128 l2 case): base vectors with integer coordinates
+0..255 from a clipped half-normal (sigma 40, ~30% zeros, seed 2), an index
+built by the reference nmslib with M=16, efConstruction=500 (its graph ships
+as a pack under workloads/, the vectors are regenerated from the seed and
+the reference-format index file is rewritten byte-for-byte, then loaded with
+loadIndex), 10,000 queries from the same distribution (seed 3), k=10,
+exact brute-force ground truth.
+
+A step is one knnQueryBatch over the 10,000 queries with queries and
+outputs resident in HBM.  The operating point is the smallest efSearch of a
+ladder whose recall@10 (tie-aware, as the reference's eval_metrics.h counts
+it) reaches 0.95.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WORKLOAD = "halfnormal1m"
+PACK = os.path.join(ROOT, "workloads", "halfnormal1m_M16_efc500.pack.npz")
+GT = os.path.join(ROOT, "workloads", "halfnormal1m_gt.npz")
+EF_LADDER = [4096, 4608, 5120, 5632, 6144, 7168, 8192, 10240, 12288]
+SWEEP = [16, 32, 64, 128, 256, 512, 1024, 2048, 4096]
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def recall_at_k(ids, gt_ids, gt_d, k=10):
+    """Tie-aware recall@k (eval_metrics.h:112-127, eval_results.h:122-128)."""
+    kth = gt_d[:, k - 1:k]
+    tie = gt_d <= kth  # ground truth is exact (integer distances)
+    tot = 0
+    for a, g, t in zip(ids[:, :k], gt_ids, tie):
+        good = set(g[t].tolist())
+        tot += sum(1 for x in a.tolist() if x in good)
+    return tot / (k * len(ids))
+
+
+def prepare_index(tmpdir):
+    from nmslib_amd import datasets as D
+    from nmslib_amd import indexpack
+
+    t = time.time()
+    pk = indexpack.load_pack(PACK)
+    x = D.base(WORKLOAD)
+    path = os.path.join(tmpdir, "halfnormal1m.hnsw")
+    indexpack.unpack(pk, x, path)
+    log(f"[bench] index file written ({os.path.getsize(path) / 1e6:.0f} MB) in {time.time() - t:.1f}s")
+    return path, x
+
+
+def cpu_baseline(path, q, gids, gd, ef, algo, gpu_ids, gpu_dists):
+    """The reference (oracle/_ref build, when present) or the oracle port timed
+    on the host cores over a bounded sample of the same queries."""
+    import ctypes as C
+
+    threads = len(os.sched_getaffinity(0))
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libnmsref.so")
+    kind = "reference" if os.path.exists(ref_so) else "port"
+    if kind == "reference":
+        sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+        from refdrv import RefIndex
+        r = RefIndex.load("l2", path, q.shape[1])
+        r.set_query_params(f"efSearch={ef},algoType={algo}")
+
+        def run(qs):
+            return r.knn_query_batch(qs, 10, num_threads=threads)
+    else:
+        from oracle.oracle import OracleIndex
+        o = OracleIndex(path)
+
+        def run(qs):
+            return o.search(qs, 10, ef, algo, num_threads=threads)
+
+    # size the sample for ~15 s of CPU work
+    t = time.time()
+    run(q[:threads * 4])
+    per_q = (time.time() - t) / (threads * 4)
+    ns = int(min(len(q), max(threads * 8, 15.0 / max(per_q, 1e-6))))
+    ns -= ns % threads or 0
+    ns = max(ns, threads)
+    t = time.time()
+    ids, dists, counts = run(q[:ns])
+    el = time.time() - t
+    same = bool(np.array_equal(ids, gpu_ids[:ns]) and np.array_equal(dists, gpu_dists[:ns]))
+    rec = recall_at_k(ids, gids[:ns], gd[:ns])
+    del run
+    return {"value": round(ns / el, 2), "unit": "queries/s", "cores": threads, "kind": kind,
+            "sample": f"first {ns} of the {len(q)} queries, efSearch={ef}, algoType={algo}, "
+                      f"k=10, {threads} threads; recall@10={rec:.4f}; results identical to the "
+                      f"GPU's: {same}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ef", type=int, default=0, help="fixed efSearch (0: smallest of the ladder "
+                    "reaching recall@10 >= 0.95)")
+    ap.add_argument("--algo", default="v1merge")
+    ap.add_argument("--storage", default="fp32", choices=["fp32", "bf16", "fp8"])
+    ap.add_argument("--nq", type=int, default=10000)
+    ap.add_argument("--sweep", action="store_true", help="also print the recall/QPS curve")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the query path "
+              f"shards by replicating the index per GPU, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import nmslib_amd as nmslib
+    from nmslib_amd import datasets as D
+
+    tmpdir = tempfile.mkdtemp(prefix="nmsamd_bench_")
+    path, _ = prepare_index(tmpdir)
+    q = D.queries(WORKLOAD, args.nq)
+    gt = np.load(GT)
+    gids, gd = gt["ids"][:args.nq], gt["dists"][:args.nq]
+    k = 10
+
+    idx = nmslib.init(method="hnsw", space="l2")
+    t = time.time()
+    idx.loadIndex(path)
+    log(f"[bench] loadIndex {time.time() - t:.1f}s {idx.info()}")
+    if args.storage != "fp32":
+        idx.setPrecision(args.storage, 0)
+    dev = torch.device("cuda:0")
+    dq = torch.from_numpy(q).to(dev)
+    dids = torch.empty((args.nq, k), dtype=torch.int32, device=dev)
+    dd = torch.empty((args.nq, k), dtype=torch.float32, device=dev)
+    dc = torch.empty((args.nq,), dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+
+    def step():
+        idx.knnQueryBatchDevice(dq.data_ptr(), args.nq, k, dids.data_ptr(), dd.data_ptr(),
+                                dc.data_ptr(), stream.cuda_stream)
+
+    def run_at(ef):
+        idx.setQueryTimeParams({"efSearch": ef, "algoType": args.algo})
+        torch.cuda.synchronize()
+        t0 = time.time()
+        step()
+        torch.cuda.synchronize()
+        el = time.time() - t0
+        return recall_at_k(dids.cpu().numpy(), gids, gd), el
+
+    curve = []
+    if args.sweep:
+        for ef in SWEEP:
+            r, el = run_at(ef)
+            curve.append({"ef": ef, "recall": round(r, 4), "qps": round(args.nq / el, 1)})
+            log(f"[bench] sweep ef={ef} recall={r:.4f} qps={args.nq / el:.0f}")
+    ef = args.ef
+    if not ef:
+        for cand in EF_LADDER:
+            r, el = run_at(cand)
+            log(f"[bench] ladder ef={cand} recall={r:.4f} ({args.nq / el:.0f} q/s)")
+            if r >= 0.95:
+                ef = cand
+                break
+        else:
+            ef = EF_LADDER[-1]
+    idx.setQueryTimeParams({"efSearch": ef, "algoType": args.algo})
+
+    # counters + per-query device latency (untimed run)
+    idx.collectStats(True)
+    step()
+    torch.cuda.synchronize()
+    st = idx.lastStats()
+    lat = idx.lastQueryTimesUs(args.nq)
+    idx.collectStats(False)
+    ids_np = dids.cpu().numpy()
+    dists_np = dd.cpu().numpy()
+    recall = recall_at_k(ids_np, gids, gd)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    kernel_ms = []
+    t0 = time.time()
+    for _ in range(args.steps):
+        step()
+        torch.cuda.synchronize()
+        kernel_ms.append(idx.lastStats()["kernel_ms"])
+    torch.cuda.synchronize()
+    elapsed = time.time() - t0
+    ms_per_step = elapsed * 1000 / args.steps
+    qps = args.nq * args.steps / elapsed
+
+    # algorithmic bytes of one launch: adjacency ints read, every distance
+    # evaluation's vector, the queries in and the results out
+    dim = q.shape[1]
+    vec_bytes = {"fp32": 4, "bf16": 2, "fp8": 1}[args.storage] * dim + (4 if args.storage == "fp8" else 0)
+    alg_bytes = (st["list_ints_read"] * 4 + (st["dist_level0"] + st["dist_upper"]) * vec_bytes
+                 + args.nq * dim * 4 + args.nq * (k * 8 + 4))
+    avg_kernel_s = float(np.mean(kernel_ms)) / 1000
+    achieved = alg_bytes / avg_kernel_s / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(path, q, gids, gd, ef, args.algo, ids_np, dists_np)
+        log(f"[bench] cpu baseline {cpu}")
+
+    out = {
+        "metric": "QPS at recall@10>=0.95 (1M x 128 L2, 1 GPU) + p50/p99 latency",
+        "value": round(qps, 1),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.storage == "fp32" else args.storage,
+        "data": "synthetic (seeded generator; reference-built index)",
+        "config": {"workload": "1,000,000 x 128 l2 half-normal u8-valued, M=16 efC=500, "
+                               "10,000 queries, k=10",
+                   "efSearch": ef, "algoType": args.algo, "storage": args.storage,
+                   "recall_at_10": round(recall, 4),
+                   "latency_us_p50": round(float(np.percentile(lat, 50)), 1),
+                   "latency_us_p99": round(float(np.percentile(lat, 99)), 1),
+                   "latency_kind": "per-query device search time inside the batch",
+                   "per_query": {"expansions": st["expansions"] / args.nq,
+                                 "dist_evals": (st["dist_level0"] + st["dist_upper"]) / args.nq,
+                                 "tie_path_expansions": st["tie_paths"] / args.nq},
+                   "curve": curve},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "hnsw_search_kernel", "avg_kernel_ms": round(avg_kernel_s * 1000, 3),
+                     "alg_bytes_per_launch": int(alg_bytes)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,115 @@
+/*
+ * nmslib_amd -- C ABI of the MI355X HNSW query path.
+ *
+ * One handle = one index object of python_bindings/nmslib.cc's
+ * IndexWrapper<float> (nmslib.cc:68-376) restricted to method "hnsw" on the
+ * dense float spaces "l2", "cosinesimil" and "negdotprod".  Every entry point
+ * returns 0 on success or a negative status; the message of the last failure
+ * on the calling thread is nmsamd_last_error().  No C++ exception crosses the
+ * ABI and the library keeps no caller pointer past a call.  Results follow the
+ * reference exactly: ids are the object ids stored in the index, distances
+ * are what the optimized search reports (squared L2 for "l2",
+ * 1 - cos for "cosinesimil", -dot for "negdotprod"), ascending.
+ */
+#ifndef NMSLIB_AMD_H
+#define NMSLIB_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nmsamd_index nmsamd_index;
+
+/* status codes */
+#define NMSAMD_OK 0
+#define NMSAMD_EINVAL (-1)   /* invalid argument   -> Python ValueError   */
+#define NMSAMD_ERUNTIME (-2) /* runtime failure    -> Python RuntimeError */
+#define NMSAMD_EIO (-3)      /* file i/o           -> Python RuntimeError */
+#define NMSAMD_EDEVICE (-4)  /* HIP device error   -> Python RuntimeError */
+
+/* Message of the last failure on this thread ("" if none). */
+const char* nmsamd_last_error(void);
+
+/* nmslib.init(space, space_params, method, data_type, dtype)
+ * (python_bindings/nmslib.cc:470-510).  Only method "hnsw", dense float data;
+ * an unknown space raises the reference's spacefactory message. */
+int nmsamd_init(const char* space, const char* method, nmsamd_index** out);
+
+/* Releases the handle and its device memory. */
+int nmsamd_free(nmsamd_index* h);
+
+/* IndexWrapper::loadIndex(filename, load_data) (nmslib.cc:98-113,
+ * Hnsw::LoadIndex hnsw.cc:993-1074): reads the optimized index file, uploads
+ * the graph and vectors to HBM, resets query-time params (ef = 20).
+ * load_data also reads filename + ".dat" (space.cc:61-86). */
+int nmsamd_load_index(nmsamd_index* h, const char* filename, int load_data);
+
+/* IndexWrapper::saveIndex(filename, save_data) (nmslib.cc:115-125): writes
+ * the optimized format byte-for-byte as Hnsw::SaveOptimizedIndex
+ * (hnsw.cc:776-806) does. */
+int nmsamd_save_index(nmsamd_index* h, const char* filename, int save_data);
+
+/* setQueryTimeParams (nmslib.cc:599-607 -> Hnsw::SetQueryTimeParams,
+ * hnsw.cc:472-507): n strings "key=value"; keys ef | efSearch, algoType
+ * (hybrid | old | v1merge), searchMethod (ignored); unknown keys raise
+ * "Unknown parameters found!".  NULL/0 resets the defaults.  Not to be called
+ * while a search on the same handle is in flight. */
+int nmsamd_set_query_time_params(nmsamd_index* h, const char* const* kv, int n);
+
+/* IndexWrapper::knnQueryBatch(queries, k, num_threads) (nmslib.cc:142-184)
+ * over host memory: q is [nq][dim] row-major float32; ids / dists are
+ * [nq][k] row-major (ascending, -1 / 0 padding past counts[i]); counts[nq].
+ * num_threads is accepted for compatibility and ignored (the batch runs on
+ * the device).  Blocks until the results are in host memory. */
+int nmsamd_knn_query_batch(nmsamd_index* h, const float* q, int64_t nq, int64_t dim, int32_t k,
+                           int32_t num_threads, int32_t* ids, float* dists, int32_t* counts);
+
+/* Same over device memory already resident in HBM; launches on `stream`
+ * (a hipStream_t, NULL = the library's own stream) and returns without
+ * waiting for completion. */
+int nmsamd_knn_query_batch_device(nmsamd_index* h, const float* d_q, int64_t nq, int32_t k,
+                                  int32_t* d_ids, float* d_dists, int32_t* d_counts,
+                                  void* stream);
+
+/* Uneven batch: per-query k[i] and ef[i] (host arrays) in one persistent
+ * launch (no reference counterpart: the reference runs one ef per index and
+ * one k per call).  Output rows have stride max_k = max(k[i]). */
+int nmsamd_knn_query_batch_uneven(nmsamd_index* h, const float* q, int64_t nq, int64_t dim,
+                                  const int32_t* k, const int32_t* ef, int32_t max_k,
+                                  int32_t* ids, float* dists, int32_t* counts);
+
+/* __len__ (nmslib.cc:348): number of data points held with the index
+ * (0 when loaded without data, as in the reference). */
+int64_t nmsamd_size(const nmsamd_index* h);
+
+/* Index facts: n, dim, maxM, maxM0, maxlevel, enterpoint, dist_func_type,
+ * current ef. */
+int nmsamd_index_info(const nmsamd_index* h, int64_t* out8);
+
+/* Search precision: 0 = FP32 vectors (reference arithmetic), 1 = BF16,
+ * 2 = FP8 e4m3 with a per-vector scale; rerank_mult > 0 reranks the best
+ * rerank_mult*k (at least k, at most the traversal capacity) candidates
+ * exactly in FP32.  No reference counterpart (extension of the query path). */
+int nmsamd_set_precision(nmsamd_index* h, int storage, int rerank_mult);
+
+/* Per-call counters of the last search (expansions, level-0 distance
+ * evaluations, upper-level distance evaluations, adjacency ints read,
+ * expansions that took the tie path) and the kernel time of the last search
+ * in milliseconds measured with HIP events on the launch stream. */
+int nmsamd_last_stats(const nmsamd_index* h, uint64_t* out5, double* kernel_ms);
+
+/* Turns counter collection on (1) or off (0); off by default.  When on, the
+ * device also records each query's search time. */
+int nmsamd_collect_stats(nmsamd_index* h, int on);
+
+/* Per-query device search time of the last search with collection on, in
+ * microseconds (first nq queries).  No reference counterpart (the
+ * reference times queries on the host, experiments.h:186-194). */
+int nmsamd_last_query_us(const nmsamd_index* h, float* out_us, int64_t nq);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

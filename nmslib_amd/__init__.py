@@ -1,0 +1,266 @@
+"""nmslib_amd -- the nmslib HNSW query path on MI355X.
+
+A drop-in for the dense-float HNSW subset of the ``nmslib`` Python module
+(python_bindings/nmslib.cc): ``init`` returns an index whose ``loadIndex``
+reads an optimized index file the reference built and saved, whose
+``setQueryTimeParams`` takes ``ef`` / ``efSearch`` / ``algoType`` and whose
+``knnQuery`` / ``knnQueryBatch`` answer on the GPU through the C ABI of
+``include/nmslib_amd.h`` (library ``nmslib_amd/libnmslib_amd.so``).
+
+    import nmslib_amd as nmslib
+    index = nmslib.init(method="hnsw", space="l2")
+    index.loadIndex("index.bin")
+    index.setQueryTimeParams({"efSearch": 100})
+    results = index.knnQueryBatch(queries, k=10)   # [(ids, dists), ...]
+
+Errors map as in the reference binding: invalid arguments raise
+``ValueError``, everything else ``RuntimeError``.  The library must be
+present: nothing here falls back to a CPU search.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import os
+
+import numpy as np
+
+__all__ = ["init", "DataType", "DistType", "FloatIndex", "library_path", "load_library"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+library_path = os.path.join(_HERE, "libnmslib_amd.so")
+
+_EINVAL = -1
+
+
+class DistType(enum.IntEnum):
+    FLOAT = 0
+    INT = 1
+
+
+class DataType(enum.IntEnum):
+    DENSE_VECTOR = 0
+    DENSE_UINT8_VECTOR = 1
+    SPARSE_VECTOR = 2
+    OBJECT_AS_STRING = 3
+
+
+_lib = None
+
+# name -> (restype, argtypes) of every entry point of include/nmslib_amd.h
+_P, _I64, _I32 = C.c_void_p, C.c_int64, C.c_int32
+SIGNATURES = {
+    "nmsamd_last_error": (C.c_char_p, []),
+    "nmsamd_init": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
+    "nmsamd_free": (C.c_int, [_P]),
+    "nmsamd_load_index": (C.c_int, [_P, C.c_char_p, C.c_int]),
+    "nmsamd_save_index": (C.c_int, [_P, C.c_char_p, C.c_int]),
+    "nmsamd_set_query_time_params": (C.c_int, [_P, C.POINTER(C.c_char_p), C.c_int]),
+    "nmsamd_knn_query_batch": (C.c_int, [_P, _P, _I64, _I64, _I32, _I32, _P, _P, _P]),
+    "nmsamd_knn_query_batch_device": (C.c_int, [_P, _P, _I64, _I32, _P, _P, _P, _P]),
+    "nmsamd_knn_query_batch_uneven": (C.c_int, [_P, _P, _I64, _I64, _P, _P, _I32, _P, _P, _P]),
+    "nmsamd_size": (_I64, [_P]),
+    "nmsamd_index_info": (C.c_int, [_P, _P]),
+    "nmsamd_set_precision": (C.c_int, [_P, C.c_int, C.c_int]),
+    "nmsamd_last_stats": (C.c_int, [_P, _P, C.POINTER(C.c_double)]),
+    "nmsamd_collect_stats": (C.c_int, [_P, C.c_int]),
+    "nmsamd_last_query_us": (C.c_int, [_P, _P, _I64]),
+}
+
+
+def load_library():
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(library_path):
+            raise RuntimeError(f"{library_path} is missing: build it with `python -m nmslib_amd.build`")
+        lib = C.CDLL(library_path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        msg = load_library().nmsamd_last_error().decode(errors="replace")
+        if rc == _EINVAL:
+            raise ValueError(msg)
+        raise RuntimeError(msg)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _params_to_kv(params):
+    """loadParams (nmslib.cc:658-692): None, a list of 'k=v' strings or a dict of
+    int / float / str values."""
+    if params is None:
+        return []
+    if isinstance(params, list):
+        return [str(p) for p in params]
+    if isinstance(params, dict):
+        out = []
+        for key, value in params.items():
+            if isinstance(value, bool) or not isinstance(value, (int, float, str)):
+                raise ValueError(f"Unknown type for parameter '{key}'")
+            out.append(f"{key}={value}")
+        return out
+    raise ValueError("Unknown type for parameters")
+
+
+def _as_queries(x, dim):
+    """readObjectVector semantics: any array-like, force-cast to C-order float32."""
+    a = np.asarray(x)
+    if a.dtype == object:
+        a = np.array([np.asarray(v, dtype=np.float32) for v in x])
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim == 1:
+        a = a.reshape(1, -1)
+    if a.ndim != 2:
+        raise ValueError("queries must be a 2-D array or a list of vectors")
+    if dim and a.shape[1] != dim:
+        raise ValueError(f"query dimension {a.shape[1]} does not match the index dimension {dim}")
+    return a
+
+
+class FloatIndex:
+    """IndexWrapper<float> for method 'hnsw' (nmslib.cc:68-376)."""
+
+    def __init__(self, space: str, method: str, data_type: DataType, dtype: DistType):
+        self.space = space
+        self.method = method
+        self.dataType = data_type
+        self.distType = dtype
+        h = C.c_void_p()
+        _check(load_library().nmsamd_init(space.encode(), method.encode(), C.byref(h)))
+        self._h = h
+        self._dim = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value and _lib is not None:
+            _lib.nmsamd_free(h)
+            self._h = None
+
+    # -- persistence ------------------------------------------------------
+    def loadIndex(self, filename: str, load_data: bool = False):
+        _check(load_library().nmsamd_load_index(self._h, os.fsencode(filename), int(bool(load_data))))
+        self._dim = self.info()["dim"]
+
+    def saveIndex(self, filename: str, save_data: bool = False):
+        _check(load_library().nmsamd_save_index(self._h, os.fsencode(filename), int(bool(save_data))))
+
+    def createIndex(self, index_params=None, print_progress=False):
+        raise RuntimeError("createIndex is not part of the device query path: build the index "
+                           "with nmslib and load it with loadIndex")
+
+    def addDataPoint(self, id, data):
+        raise RuntimeError("addDataPoint is not part of the device query path")
+
+    def addDataPointBatch(self, data, ids=None):
+        raise RuntimeError("addDataPointBatch is not part of the device query path")
+
+    # -- query-time parameters -------------------------------------------
+    def setQueryTimeParams(self, params=None):
+        kv = _params_to_kv(params)
+        arr = (C.c_char_p * max(1, len(kv)))(*[s.encode() for s in kv])
+        _check(load_library().nmsamd_set_query_time_params(self._h, arr, len(kv)))
+
+    def setPrecision(self, storage: str = "fp32", rerank_mult: int = 0):
+        """Vector storage used while traversing ('fp32', 'bf16', 'fp8') and the
+        FP32 rerank width as a multiple of k (0 = no rerank)."""
+        code = {"fp32": 0, "bf16": 1, "fp8": 2}[storage]
+        _check(load_library().nmsamd_set_precision(self._h, code, int(rerank_mult)))
+
+    def collectStats(self, on: bool = True):
+        _check(load_library().nmsamd_collect_stats(self._h, int(bool(on))))
+
+    def lastStats(self):
+        out = np.zeros(5, np.uint64)
+        ms = C.c_double()
+        _check(load_library().nmsamd_last_stats(self._h, _ptr(out), C.byref(ms)))
+        keys = ("expansions", "dist_level0", "dist_upper", "list_ints_read", "tie_paths")
+        d = {k: int(v) for k, v in zip(keys, out)}
+        d["kernel_ms"] = ms.value
+        return d
+
+    def lastQueryTimesUs(self, nq: int):
+        """Per-query device search time (us) of the last search made with
+        collectStats(True)."""
+        out = np.empty(nq, np.float32)
+        _check(load_library().nmsamd_last_query_us(self._h, _ptr(out), int(nq)))
+        return out
+
+    def info(self):
+        out = np.zeros(8, np.int64)
+        _check(load_library().nmsamd_index_info(self._h, _ptr(out)))
+        keys = ("n", "dim", "maxM", "maxM0", "maxlevel", "enterpoint", "dist_func_type", "ef")
+        return {k: int(v) for k, v in zip(keys, out)}
+
+    # -- search -------------------------------------------------------------
+    def knnQueryBatchArrays(self, queries, k: int = 10):
+        """Batch search returning (ids[nq, k], dists[nq, k], counts[nq])."""
+        q = _as_queries(queries, self._dim)
+        nq = q.shape[0]
+        ids = np.empty((nq, k), np.int32)
+        dists = np.empty((nq, k), np.float32)
+        counts = np.empty(nq, np.int32)
+        _check(load_library().nmsamd_knn_query_batch(self._h, _ptr(q), nq, q.shape[1], int(k), 0,
+                                                     _ptr(ids), _ptr(dists), _ptr(counts)))
+        return ids, dists, counts
+
+    def knnQueryBatch(self, queries, k: int = 10, num_threads: int = 0):
+        """knnQueryBatch (nmslib.cc:142-184): a list of (ids int32, dists float32)
+        per query, ascending.  num_threads is accepted and ignored."""
+        ids, dists, counts = self.knnQueryBatchArrays(queries, k)
+        return [(ids[i, :c].copy(), dists[i, :c].copy()) for i, c in enumerate(counts)]
+
+    def knnQuery(self, vector, k: int = 10):
+        ids, dists = self.knnQueryBatch(np.asarray(vector, np.float32).reshape(1, -1), k)[0]
+        return ids, dists
+
+    def knnQueryBatchUneven(self, queries, ks, efs):
+        """One persistent launch with per-query k and ef."""
+        q = _as_queries(queries, self._dim)
+        nq = q.shape[0]
+        ks = np.ascontiguousarray(ks, np.int32)
+        efs = np.ascontiguousarray(efs, np.int32)
+        if ks.shape != (nq,) or efs.shape != (nq,):
+            raise ValueError("ks and efs need one entry per query")
+        mk = int(ks.max()) if nq else 0
+        ids = np.empty((nq, max(mk, 1)), np.int32)
+        dists = np.empty((nq, max(mk, 1)), np.float32)
+        counts = np.empty(nq, np.int32)
+        _check(load_library().nmsamd_knn_query_batch_uneven(
+            self._h, _ptr(q), nq, q.shape[1], _ptr(ks), _ptr(efs), max(mk, 1),
+            _ptr(ids), _ptr(dists), _ptr(counts)))
+        return ids, dists, counts
+
+    def knnQueryBatchDevice(self, q_ptr: int, nq: int, k: int, ids_ptr: int, dists_ptr: int,
+                            counts_ptr: int, stream: int = 0):
+        """Queries and outputs already in device memory (raw pointers)."""
+        _check(load_library().nmsamd_knn_query_batch_device(
+            self._h, C.c_void_p(q_ptr), int(nq), int(k), C.c_void_p(ids_ptr),
+            C.c_void_p(dists_ptr), C.c_void_p(counts_ptr), C.c_void_p(stream or None)))
+
+    def __len__(self):
+        return int(load_library().nmsamd_size(self._h))
+
+    def __repr__(self):
+        return f"<nmslib_amd.FloatIndex method='{self.method}' space='{self.space}'>"
+
+
+def init(space: str = "cosinesimil", space_params=None, method: str = "hnsw",
+         data_type: DataType = DataType.DENSE_VECTOR, dtype: DistType = DistType.FLOAT):
+    """nmslib.init (nmslib.cc:470-510) for dense float HNSW indices."""
+    if space_params:
+        raise ValueError("space parameters are not used by the l2 / cosinesimil / negdotprod spaces")
+    if DataType(data_type) != DataType.DENSE_VECTOR:
+        raise ValueError("only DataType.DENSE_VECTOR is supported by the device query path")
+    if DistType(dtype) != DistType.FLOAT:
+        raise ValueError("only DistType.FLOAT is supported by the device query path")
+    return FloatIndex(space, method, DataType(data_type), DistType(dtype))

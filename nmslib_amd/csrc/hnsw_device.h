@@ -1,0 +1,89 @@
+// Device-resident HNSW index and launch arguments shared by the host library
+// and the HIP kernels.
+//
+// HBM layout of a loaded optimized index (one copy per device, read-only
+// while searching):
+//   vec     [n][dstride] f32   level-0 vectors as stored in the index file
+//                              (cosine ones already normalised there); rows
+//                              are padded with zeros to a multiple of 4 floats
+//                              so every row starts on a 16-byte boundary
+//   vec16   [n][dstride] bf16  optional reduced-precision copy
+//   vec8    [n][dstride] fp8   optional e4m3 copy, with scale8[n] f32
+//   l0      [n][l0stride] i32  level-0 adjacency, the list's ids in stored
+//                              order followed by -1 up to the padded stride
+//                              (l0stride = maxM0 rounded up to 32 -> 128 B rows)
+//   up_off  [n] i32            offset of a node's upper-level lists in `up`,
+//                              -1 for level-0-only nodes
+//   up      flat i32           per node, per level L>=1: [count, ids[maxM]]
+//                              (the file's own per-level blocks)
+//   ext_id  [n] i32            object id stored in each element (results
+//                              report these, hnsw_distfunc_opt.cc:280)
+// Per search slot (a resident wavefront):
+//   visited [slots][vstride] u8  epoch marks, as the reference's VisitedList
+//   epoch   [slots] u8           current epoch of each slot
+#ifndef NMSLIB_AMD_HNSW_DEVICE_H
+#define NMSLIB_AMD_HNSW_DEVICE_H
+#include <stdint.h>
+
+namespace nmsamd {
+
+// distance function codes of the optimized index (include/method/hnsw.h:50-58)
+enum DistFunc : int { kL2Sqr16 = 1, kL2Sqr = 2, kNormCos = 3, kNegDot = 4 };
+// vector storage used while traversing
+enum Storage : int { kF32 = 0, kBF16 = 1, kFP8 = 2 };
+
+constexpr int kWave = 64;
+constexpr int kWavesPerBlock = 4;
+constexpr int kMaxM0 = 64;          // widest level-0 list one expansion handles
+constexpr uint32_t kUsedBit = 0x80000000u;
+
+struct DevIndex {
+  const float* vec;
+  const uint16_t* vec16;
+  const uint8_t* vec8;
+  const float* scale8;
+  const int32_t* l0;
+  const int32_t* up_off;
+  const int32_t* up;
+  const int32_t* ext_id;
+  uint8_t* visited;
+  uint8_t* epoch;
+  int64_t n;
+  int64_t vstride;
+  int dim, dstride, l0stride, maxM, maxM0, maxlevel, enter, func;
+  int has_dups;
+};
+
+struct SearchStats {
+  unsigned long long expansions;
+  unsigned long long dist_level0;
+  unsigned long long dist_upper;
+  unsigned long long list_ints_read;
+  unsigned long long tie_paths;
+};
+
+struct SearchArgs {
+  DevIndex ix;
+  const float* queries;      // [nq][dim]
+  int64_t nq;
+  const int32_t* k_of;       // optional per-query k
+  const int32_t* ef_of;      // optional per-query ef
+  int k, ef;                 // used when the per-query arrays are null
+  int cap;                   // LDS capacity of the sorted array (>= max(ef,k) of every query)
+  int out_stride;            // row stride of out_ids / out_dists
+  int32_t* out_ids;
+  float* out_dists;
+  int32_t* out_counts;
+  int32_t* out_internal;     // optional [nq][out_stride] internal node ids
+  uint32_t* q_ticks;         // optional [nq] per-query search time, 100 MHz ticks
+  int* work;                 // work counter, zero before the launch
+  SearchStats* stats;        // optional
+  int storage;               // Storage
+  int algo_old;              // 1: SearchOld semantics
+  float* old_heap_key;       // SearchOld scratch per slot
+  int32_t* old_heap_id;
+  int64_t old_heap_cap;
+};
+
+}  // namespace nmsamd
+#endif

@@ -1,0 +1,707 @@
+// HNSW k-NN search kernels for CDNA4 (gfx950): one wavefront per query.
+//
+// Semantics follow the reference's optimized search
+// (similarity_search/src/method/hnsw_distfunc_opt.cc):
+//   * greedy descent over the upper levels (:168-198): a node moves to a
+//     strictly closer neighbour, scanning the list in stored order;
+//   * SearchV1Merge at level 0 (:200-281): a bounded sorted array of
+//     capacity max(ef, k), one candidate expanded per step, new items sorted
+//     and inserted with push_or_replace_non_empty_exp (sort_arr_bi.h:159-199);
+//   * SearchOld (:46-150) with its two binary heaps, for ef >= 1000 or
+//     algoType=old;
+//   * the first min(k, size) items reported in (distance, object) order
+//     (knnqueue.h:73-74; objects are ordered by their internal index).
+//
+// MI355X mapping:
+//   * a lane pair owns one candidate vector: lane h of the pair loads float4s
+//     [16c+4h, 16c+4h+4) and [16c+8+4h, ...) of each 16-float chunk, i.e. the
+//     four AVX accumulator lanes 4h..4h+3 of the reference kernel, so the
+//     FP32 distance is computed in the reference's exact summation order
+//     (hnsw_distfunc_opt_impl_inline.h:42-173) while 32 vectors stream per
+//     pass (16 KB of 128-d vectors in flight per wave);
+//   * the sorted array, the query and the per-expansion buffers live in LDS;
+//     new items are merged in parallel (rank by binary search) and only an
+//     expansion whose keys tie an existing key falls back to item-by-item
+//     insertion that replays the reference's exponential-search window;
+//   * the visited set is the reference's epoch-byte array, one per resident
+//     wave, in HBM;
+//   * the grid is persistent: waves pull queries from an atomic counter, so
+//     queries of uneven ef / k balance across the device.
+//
+// This file is compiled with -ffp-contract=off: FP32 distances must not be
+// contracted into FMAs (the reference rounds the product before the add).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hnsw_device.h"
+#include "sort_emul.h"
+
+namespace nmsamd {
+
+#define WSYNC()                                               \
+  do {                                                        \
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");    \
+    __builtin_amdgcn_wave_barrier();                          \
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");    \
+  } while (0)
+
+__device__ __forceinline__ float kInf() { return __builtin_inff(); }
+
+template <int FUNC>
+__device__ __forceinline__ float term(float q, float x) {
+  if constexpr (FUNC == kL2Sqr16 || FUNC == kL2Sqr) {
+    const float t = q - x;
+    return t * t;
+  } else {
+    return q * x;
+  }
+}
+
+template <int FUNC>
+__device__ __forceinline__ float finish(float s) {
+  if constexpr (FUNC == kNormCos) {
+    // max(0, 1 - max(-1, min(1, dot)))   (hnsw.cc:78-81)
+    const float c = s < -1.0f ? -1.0f : (s > 1.0f ? 1.0f : s);
+    const float r = 1.0f - c;
+    return r < 0.0f ? 0.0f : r;
+  } else if constexpr (FUNC == kNegDot) {
+    return -s;
+  } else {
+    return s;
+  }
+}
+
+// FP32 distance of the query (LDS) to one stored vector, computed by a lane
+// pair in the AVX order of the reference; both lanes return the same value.
+template <int D, int FUNC>
+__device__ __forceinline__ float dist_pair_f32(const float* __restrict__ qs,
+                                               const float* __restrict__ x, int h, int drt) {
+  const int d = D > 0 ? D : drt;
+  const int q16 = d >> 4;
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+#pragma unroll
+  for (int c = 0; c < q16; ++c) {
+    const float4 x0 = *reinterpret_cast<const float4*>(x + 16 * c + 4 * h);
+    const float4 x1 = *reinterpret_cast<const float4*>(x + 16 * c + 8 + 4 * h);
+    const float4 q0 = *reinterpret_cast<const float4*>(qs + 16 * c + 4 * h);
+    const float4 q1 = *reinterpret_cast<const float4*>(qs + 16 * c + 8 + 4 * h);
+    a0 = a0 + term<FUNC>(q0.x, x0.x);
+    a1 = a1 + term<FUNC>(q0.y, x0.y);
+    a2 = a2 + term<FUNC>(q0.z, x0.z);
+    a3 = a3 + term<FUNC>(q0.w, x0.w);
+    a0 = a0 + term<FUNC>(q1.x, x1.x);
+    a1 = a1 + term<FUNC>(q1.y, x1.y);
+    a2 = a2 + term<FUNC>(q1.z, x1.z);
+    a3 = a3 + term<FUNC>(q1.w, x1.w);
+  }
+  const float p0 = __shfl_xor(a0, 1), p1 = __shfl_xor(a1, 1);
+  const float p2 = __shfl_xor(a2, 1), p3 = __shfl_xor(a3, 1);
+  // accumulators 0..3 live in the even lane, 4..7 in the odd one
+  const float l0 = h ? p0 : a0, l1 = h ? p1 : a1, l2 = h ? p2 : a2, l3 = h ? p3 : a3;
+  const float u0 = h ? a0 : p0, u1 = h ? a1 : p1, u2 = h ? a2 : p2, u3 = h ? a3 : p3;
+  float s;
+  if constexpr (FUNC == kL2Sqr16) {
+    s = l0 + l1;
+    s = s + l2;
+    s = s + l3;
+    s = s + u0;
+    s = s + u1;
+    s = s + u2;
+    s = s + u3;
+  } else {
+    float s0 = l0 + u0, s1 = l1 + u1, s2 = l2 + u2, s3 = l3 + u3;
+    int c = 16 * q16;
+    for (; c + 4 <= d; c += 4) {
+      const float4 x4 = *reinterpret_cast<const float4*>(x + c);
+      const float4 q4 = *reinterpret_cast<const float4*>(qs + c);
+      s0 = s0 + term<FUNC>(q4.x, x4.x);
+      s1 = s1 + term<FUNC>(q4.y, x4.y);
+      s2 = s2 + term<FUNC>(q4.z, x4.z);
+      s3 = s3 + term<FUNC>(q4.w, x4.w);
+    }
+    s = s0 + s1;
+    s = s + s2;
+    s = s + s3;
+    for (; c < d; ++c) s = s + term<FUNC>(qs[c], x[c]);
+  }
+  return finish<FUNC>(s);
+}
+
+__device__ __forceinline__ float bf16_to_f32(uint32_t bits16) {
+  return __uint_as_float(bits16 << 16);
+}
+
+// Reduced-precision distance (BF16 or per-vector-scaled FP8 e4m3 storage,
+// FP32 accumulation).  A lane pair splits every 16-element chunk into two
+// halves; the order is not the reference's (the stored values are not the
+// reference's either) -- the exact FP32 rerank restores reference distances.
+template <int D, int FUNC, int ST>
+__device__ __forceinline__ float dist_pair_lp(const float* __restrict__ qs, const DevIndex& ix,
+                                              int64_t node, int h, int drt) {
+  const int d = D > 0 ? D : drt;
+  const int q16 = d >> 4;
+  float acc = 0.f;
+  float scale = 1.f;
+  if constexpr (ST == kFP8) scale = ix.scale8[node];
+#pragma unroll
+  for (int c = 0; c < q16; ++c) {
+    const int e0 = 16 * c + 8 * h;
+    float xv[8];
+    if constexpr (ST == kBF16) {
+      const uint4 raw =
+          *reinterpret_cast<const uint4*>(ix.vec16 + node * ix.dstride + e0);
+      const uint32_t w[4] = {raw.x, raw.y, raw.z, raw.w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        xv[2 * j] = bf16_to_f32(w[j] & 0xffffu);
+        xv[2 * j + 1] = bf16_to_f32(w[j] >> 16);
+      }
+    } else {
+      const uint2 raw = *reinterpret_cast<const uint2*>(ix.vec8 + node * ix.dstride + e0);
+      const uint32_t w[2] = {raw.x, raw.y};
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const auto lo = __builtin_amdgcn_cvt_pk_f32_fp8((int)w[j], false);
+        const auto hi = __builtin_amdgcn_cvt_pk_f32_fp8((int)w[j], true);
+        xv[4 * j] = lo[0] * scale;
+        xv[4 * j + 1] = lo[1] * scale;
+        xv[4 * j + 2] = hi[0] * scale;
+        xv[4 * j + 3] = hi[1] * scale;
+      }
+    }
+    const float4 qa = *reinterpret_cast<const float4*>(qs + e0);
+    const float4 qb = *reinterpret_cast<const float4*>(qs + e0 + 4);
+    const float qv[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if constexpr (FUNC == kL2Sqr16 || FUNC == kL2Sqr) {
+        const float t = qv[j] - xv[j];
+        acc = __builtin_fmaf(t, t, acc);
+      } else {
+        acc = __builtin_fmaf(qv[j], xv[j], acc);
+      }
+    }
+  }
+  for (int c = 16 * q16 + h; c < d; c += 2) {
+    float xv;
+    if constexpr (ST == kBF16) {
+      xv = bf16_to_f32(ix.vec16[node * ix.dstride + c]);
+    } else {
+      const auto p = __builtin_amdgcn_cvt_pk_f32_fp8((int)ix.vec8[node * ix.dstride + c], false);
+      xv = p[0] * scale;
+    }
+    if constexpr (FUNC == kL2Sqr16 || FUNC == kL2Sqr) {
+      const float t = qs[c] - xv;
+      acc = __builtin_fmaf(t, t, acc);
+    } else {
+      acc = __builtin_fmaf(qs[c], xv, acc);
+    }
+  }
+  acc = acc + __shfl_xor(acc, 1);
+  return finish<FUNC>(acc);
+}
+
+template <int D, int FUNC, int ST>
+__device__ __forceinline__ float dist_node(const float* __restrict__ qs, const DevIndex& ix,
+                                           int64_t node, int h) {
+  if constexpr (ST == kF32) {
+    return dist_pair_f32<D, FUNC>(qs, ix.vec + node * ix.dstride, h, ix.dim);
+  } else {
+    return dist_pair_lp<D, FUNC, ST>(qs, ix, node, h, ix.dim);
+  }
+}
+
+// number of keys[0..n) < v (lower) or <= v (upper), by binary search
+__device__ __forceinline__ int lower_count(const float* keys, int n, float v) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (keys[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+__device__ __forceinline__ int upper_count(const float* keys, int n, float v) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (keys[mid] <= v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+struct WaveLds {
+  float* keys;     // [cap] sorted array keys
+  uint32_t* ids;   // [cap] internal ids | kUsedBit
+  float* qs;       // [dstride] query
+  int* cand;       // [64] fresh neighbours of this expansion (list order)
+  float* cdist;    // [64] their distances
+  float* bkey;     // [64] buffer (list order)
+  int* bid;
+  float* skey;     // [64] buffer (sorted)
+  int* sid;
+};
+
+__device__ __forceinline__ WaveLds carve(unsigned char* base, int cap, int dstride) {
+  WaveLds w;
+  w.keys = reinterpret_cast<float*>(base);
+  w.ids = reinterpret_cast<uint32_t*>(w.keys + cap);
+  w.qs = reinterpret_cast<float*>(w.ids + cap);
+  w.cand = reinterpret_cast<int*>(w.qs + dstride);
+  w.cdist = reinterpret_cast<float*>(w.cand + 64);
+  w.bkey = w.cdist + 64;
+  w.bid = reinterpret_cast<int*>(w.bkey + 64);
+  w.skey = reinterpret_cast<float*>(w.bid + 64);
+  w.sid = reinterpret_cast<int*>(w.skey + 64);
+  return w;
+}
+
+// Greedy descent over levels maxlevel..1 (hnsw_distfunc_opt.cc:168-198).  A
+// full scan of the list is equivalent to taking the first position holding
+// the list's minimum when that minimum is below the current distance.
+template <int D, int FUNC, int ST>
+__device__ __forceinline__ void greedy_descent(const DevIndex& ix, const float* qs, int lane,
+                                               int h, int& cur, float& curd,
+                                               unsigned long long& nd, unsigned long long& nl) {
+  for (int lev = ix.maxlevel; lev > 0; --lev) {
+    for (;;) {
+      const int off = ix.up_off[cur];
+      if (off < 0) break;  // a malformed index cannot send the scan out of bounds
+      const int32_t* lst = ix.up + off + (ix.maxM + 1) * (lev - 1);
+      const int cnt = min(lst[0], ix.maxM);
+      nd += cnt;
+      nl += 1 + cnt;
+      float bestd = kInf();
+      int bestp = 0x7fffffff;
+      for (int b = 0; b < cnt; b += 32) {
+        const int s = b + (lane >> 1);
+        const bool ok = s < cnt;
+        const int t = ok ? lst[1 + s] : cur;
+        const float dd = dist_node<D, FUNC, ST>(qs, ix, t, h);
+        float kd = (ok && h == 0) ? dd : kInf();
+        int kp = (ok && h == 0) ? s : 0x7fffffff;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+          const float od = __shfl_xor(kd, o);
+          const int op = __shfl_xor(kp, o);
+          if (od < kd || (od == kd && op < kp)) {
+            kd = od;
+            kp = op;
+          }
+        }
+        if (kd < bestd) {  // earlier passes win ties (smaller positions)
+          bestd = kd;
+          bestp = kp;
+        }
+      }
+      if (bestd < curd) {
+        curd = bestd;
+        cur = lst[1 + bestp];
+      } else {
+        break;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SearchV1Merge at level 0 for one query held by one wave.
+template <int D, int FUNC, int ST>
+__device__ void v1merge_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, uint8_t ep,
+                              int qi, int k, int ef, int lane, int h, uint64_t ltm) {
+  const DevIndex& ix = a.ix;
+  const int C = max(ef, k);
+  unsigned long long nexp = 0, nd0 = 0, ndu = 1, nl = 0, nties = 0;
+
+  int cur = ix.enter;
+  float curd = dist_node<D, FUNC, ST>(w.qs, ix, cur, h);
+  greedy_descent<D, FUNC, ST>(ix, w.qs, lane, h, cur, curd, ndu, nl);
+
+  int n = 1;
+  if (lane == 0) {
+    w.keys[0] = curd;
+    w.ids[0] = (uint32_t)cur;
+    vis[cur] = ep;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  int ce = 0;
+  while (ce < min(n, ef)) {
+    WSYNC();
+    const uint32_t e = w.ids[ce];
+    const int node = (int)(e & ~kUsedBit);
+    if (lane == 0) w.ids[ce] = e | kUsedBit;
+    ++ce;
+    ++nexp;
+    const float topk = w.keys[n - 1];
+    const int nsnap = n;
+
+    // adjacency list (one 128-byte row for maxM0 <= 32)
+    const int32_t* lst = ix.l0 + (int64_t)node * ix.l0stride;
+    const int t = lane < ix.maxM0 ? lst[lane] : -1;
+    const bool valid = t >= 0;
+    nl += 1 + __popcll(__ballot(valid));
+    bool fresh = false;
+    if (valid) fresh = vis[t] != ep;
+    if (ix.has_dups) {
+      w.cand[lane] = t;
+      WSYNC();
+      for (int j = 0; j < lane; ++j)
+        if (w.cand[j] == t) fresh = false;
+      WSYNC();
+    }
+    if (fresh) vis[t] = ep;
+    const uint64_t fm = __ballot(fresh);
+    const int ncand = __popcll(fm);
+    if (fresh) w.cand[__popcll(fm & ltm)] = t;
+    nd0 += ncand;
+    WSYNC();
+    for (int b = 0; b < ncand; b += 32) {
+      const int s = b + (lane >> 1);
+      const bool ok = s < ncand;
+      const int id = ok ? w.cand[s] : node;
+      const float dd = dist_node<D, FUNC, ST>(w.qs, ix, id, h);
+      if (ok && h == 0) w.cdist[s] = dd;
+    }
+    WSYNC();
+    const float dj = lane < ncand ? w.cdist[lane] : 0.f;
+    const bool keep = lane < ncand && (dj < topk || nsnap < ef);
+    const uint64_t km = __ballot(keep);
+    const int m = __popcll(km);
+    if (m > 0) {
+      if (keep) {
+        const int r = __popcll(km & ltm);
+        w.bkey[r] = dj;
+        w.bid[r] = w.cand[lane];
+      }
+      WSYNC();
+      // stable ranks; an equal pair among more than 16 items needs the
+      // exact std::sort order (insertion sort is stable up to 16 items)
+      const float mk = lane < m ? w.bkey[lane] : 0.f;
+      const int mid = lane < m ? w.bid[lane] : 0;
+      int rank = 0;
+      bool tie = false;
+      if (lane < m) {
+        for (int j = 0; j < m; ++j) {
+          const float kj = w.bkey[j];
+          rank += (kj < mk) || (kj == mk && j < lane);
+          tie |= (kj == mk) && (j != lane);
+        }
+      }
+      const bool anytie = __ballot(tie) != 0;
+      if (anytie && m > 16) {
+        if (lane < m) {
+          w.skey[lane] = mk;
+          w.sid[lane] = mid;
+        }
+        WSYNC();
+        if (lane == 0) emu_std_sort(w.skey, w.sid, m);
+      } else if (lane < m) {
+        w.skey[rank] = mk;
+        w.sid[rank] = mid;
+      }
+      WSYNC();
+      const float bj = lane < m ? w.skey[lane] : 0.f;
+      const int bidj = lane < m ? w.sid[lane] : 0;
+      const int Lj = lane < m ? lower_count(w.keys, n, bj) : 0;
+      const int Uj = lane < m ? upper_count(w.keys, n, bj) : 0;
+      const bool slow = anytie || (__ballot(lane < m && Lj != Uj) != 0);
+      if (!slow) {
+        // distinct keys: sequential insertion == merge truncated to C
+        const int L0 = __shfl(Lj, 0);
+        const int nn = min(C, n + m);
+        for (int top = ((n - 1) >> 6) << 6; top >= ((L0 >> 6) << 6); top -= 64) {
+          const int i = top + lane;
+          const bool mv = i >= L0 && i < n;
+          const float ki = mv ? w.keys[i] : 0.f;
+          const uint32_t ii = mv ? w.ids[i] : 0u;
+          const int np = mv ? i + lower_count(w.skey, m, ki) : 0;
+          WSYNC();
+          if (mv && np < C) {
+            w.keys[np] = ki;
+            w.ids[np] = ii;
+          }
+          WSYNC();
+        }
+        if (lane < m && lane + Lj < C) {
+          w.keys[lane + Lj] = bj;
+          w.ids[lane + Lj] = (uint32_t)bidj;
+        }
+        n = nn;
+        if (L0 < ce) ce = L0;
+      } else {
+        // Equal keys: replay the reference's item-by-item insertion.  The
+        // items already inserted (S, at most 64) stay in registers, lane t
+        // holding S[t] with the count of surviving A items before it; the
+        // live array is the interleaving of A[0..na) and S, so each step
+        // only needs counts over A (precomputed) and ballots over S, and A is
+        // moved once at the end.
+        ++nties;
+        float s_key = 0.f;
+        int s_id = 0, s_ab = 0;
+        int na = n, ns = 0;
+        for (int j = 0; j < m; ++j) {
+          const float b = __shfl(bj, j);
+          const int bi = __shfl(bidj, j);
+          const int LA = min(__shfl(Lj, j), na);
+          const int UA = min(__shfl(Uj, j), na);
+          const int nv = na + ns;
+          const int ab_last = __shfl(s_ab, ns > 0 ? ns - 1 : 0);
+          const bool last_is_s = ns > 0 && ab_last == na;
+          const float lastkey = last_is_s ? __shfl(s_key, ns - 1) : w.keys[na - 1];
+          int ret;
+          if (lastkey <= b) {
+            // append after an equal last key while there is room
+            if (nv < C) {
+              if (lane == ns) {
+                s_key = b;
+                s_ab = na;
+                s_id = bi;
+              }
+              ++ns;
+            }
+            ret = nv;
+          } else {
+            const int L = LA + __popcll(__ballot(lane < ns && s_key < b));
+            const int U = UA + __popcll(__ballot(lane < ns && s_key <= b));
+            // exponential walk down from the last item (sort_arr_bi.h:170-181),
+            // then the lower bound inside the last step's window
+            int c = nv - 1, prev = c, step = 1;
+            while (c > 0 && c >= U) {
+              prev = c;
+              c -= step;
+              step *= 2;
+              if (step > c) step = c;
+            }
+            const int pos = c < prev ? max(c, L) : c;
+            if (nv == C) {  // the last item falls off
+              if (last_is_s) --ns; else --na;
+            }
+            const int t0 = __popcll(__ballot(lane < ns && s_ab + lane < pos));
+            const float uk = __shfl_up(s_key, 1);
+            const int ua = __shfl_up(s_ab, 1);
+            const int ui = __shfl_up(s_id, 1);
+            if (lane > t0 && lane <= ns) {
+              s_key = uk;
+              s_ab = ua;
+              s_id = ui;
+            }
+            if (lane == t0) {
+              s_key = b;
+              s_ab = pos - t0;
+              s_id = bi;
+            }
+            ++ns;
+            ret = pos;
+          }
+          if (ret < ce) ce = ret;
+        }
+        // materialise: A item i moves up by the number of S items before it
+        if (lane < ns) {
+          w.bkey[lane] = s_key;
+          w.bid[lane] = s_id;
+          w.cand[lane] = s_ab;
+        }
+        WSYNC();
+        const int ab0 = ns > 0 ? __shfl(s_ab, 0) : na;
+        for (int top = ((na - 1) >> 6) << 6; top >= ((ab0 >> 6) << 6); top -= 64) {
+          const int i = top + lane;
+          const bool mv = i >= ab0 && i < na;
+          const float ki = mv ? w.keys[i] : 0.f;
+          const uint32_t ii = mv ? w.ids[i] : 0u;
+          int before = 0;
+          if (mv) {
+            int lo = 0, hi = ns;
+            while (lo < hi) {
+              const int mid = (lo + hi) >> 1;
+              if (w.cand[mid] <= i) lo = mid + 1; else hi = mid;
+            }
+            before = lo;
+          }
+          WSYNC();
+          if (mv) {
+            w.keys[i + before] = ki;
+            w.ids[i + before] = ii;
+          }
+          WSYNC();
+        }
+        if (lane < ns) {
+          w.keys[s_ab + lane] = s_key;
+          w.ids[s_ab + lane] = (uint32_t)s_id;
+        }
+        n = na + ns;
+      }
+    }
+    WSYNC();
+    // the next unexpanded item (hnsw_distfunc_opt.cc:272-273)
+    while (ce < n) {
+      const int i = ce + lane;
+      const bool unused = i < n && !(w.ids[i] & kUsedBit);
+      const uint64_t um = __ballot(unused);
+      if (um) {
+        ce += __ffsll((long long)um) - 1;
+        break;
+      }
+      ce = min(ce + 64, n);
+    }
+    // the marks of this expansion must land before the next one reads them
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  }
+  WSYNC();
+
+  // report the first min(k, n) items, ties by internal index
+  const int kk = min(k, n);
+  const int64_t row = (int64_t)qi * a.out_stride;
+  for (int b = 0; b < k; b += 64) {
+    const int p = b + lane;
+    if (p < kk) {
+      const float key = w.keys[p];
+      const uint32_t id = w.ids[p] & ~kUsedBit;
+      const int start = lower_count(w.keys, kk, key);
+      int rank = start;
+      for (int j = start; j < kk && w.keys[j] == key; ++j)
+        rank += ((w.ids[j] & ~kUsedBit) < id) ? 1 : 0;
+      a.out_ids[row + rank] = ix.ext_id[id];
+      a.out_dists[row + rank] = key;
+      if (a.out_internal) a.out_internal[row + rank] = (int32_t)id;
+    } else if (p < k) {
+      a.out_ids[row + p] = -1;
+      a.out_dists[row + p] = 0.f;
+      if (a.out_internal) a.out_internal[row + p] = -1;
+    }
+  }
+  if (lane == 0) {
+    a.out_counts[qi] = kk;
+    if (a.stats) {
+      atomicAdd(&a.stats->expansions, nexp);
+      atomicAdd(&a.stats->dist_level0, nd0);
+      atomicAdd(&a.stats->dist_upper, ndu);
+      atomicAdd(&a.stats->list_ints_read, nl);
+      atomicAdd(&a.stats->tie_paths, nties);
+    }
+  }
+}
+
+template <int D, int FUNC, int ST>
+__global__ void __launch_bounds__(kWave* kWavesPerBlock)
+    hnsw_search_kernel(SearchArgs a, int lds_wave_bytes) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int h = lane & 1;
+  const uint64_t ltm = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const WaveLds w = carve(smem + wave * lds_wave_bytes, a.cap, a.ix.dstride);
+  const int64_t slot = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+  uint8_t* vis = a.ix.visited + slot * a.ix.vstride;
+  const int dim = a.ix.dim;
+
+  for (;;) {
+    int qi = 0;
+    if (lane == 0) qi = atomicAdd(a.work, 1);
+    qi = __shfl(qi, 0);
+    if (qi >= a.nq) break;
+    const int k = a.k_of ? a.k_of[qi] : a.k;
+    const int ef = a.ef_of ? a.ef_of[qi] : a.ef;
+    const uint64_t t_start = __builtin_amdgcn_s_memrealtime();
+
+    // a fresh epoch for this slot's visited marks (hnsw.h:576-583)
+    int epi = 0;
+    if (lane == 0) epi = a.ix.epoch[slot];
+    epi = __shfl(epi, 0);
+    uint8_t ep = (uint8_t)(epi + 1);
+    if (ep == 0) {
+      uint4* v4 = reinterpret_cast<uint4*>(vis);
+      const int64_t nv4 = a.ix.vstride / 16;
+      for (int64_t i = lane; i < nv4; i += 64) v4[i] = make_uint4(0, 0, 0, 0);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      ep = 1;
+    }
+    if (lane == 0) a.ix.epoch[slot] = ep;
+
+    // the query, normalised in place for cosine (hnsw.h:486-497)
+    const float* qg = a.queries + (int64_t)qi * dim;
+    for (int i = lane; i < a.ix.dstride; i += 64) w.qs[i] = i < dim ? qg[i] : 0.f;
+    WSYNC();
+    if constexpr (FUNC == kNormCos) {
+      float inv = 0.f;
+      if (lane == 0) {
+        float sum = 0.f;
+        for (int i = 0; i < dim; ++i) sum = sum + w.qs[i] * w.qs[i];
+        inv = sum != 0.0f ? 1.0f / sqrtf(sum) : 0.f;
+      }
+      inv = __shfl(inv, 0);
+      if (inv != 0.f)
+        for (int i = lane; i < dim; i += 64) w.qs[i] = w.qs[i] * inv;
+      WSYNC();
+    }
+    if (k <= 0 || a.ix.n == 0) {
+      for (int p = lane; p < k; p += 64) {
+        a.out_ids[(int64_t)qi * a.out_stride + p] = -1;
+        a.out_dists[(int64_t)qi * a.out_stride + p] = 0.f;
+      }
+      if (lane == 0) a.out_counts[qi] = 0;
+      continue;
+    }
+    v1merge_query<D, FUNC, ST>(a, w, vis, ep, qi, k, ef, lane, h, ltm);
+    WSYNC();
+    if (a.q_ticks && lane == 0)
+      a.q_ticks[qi] = (uint32_t)(__builtin_amdgcn_s_memrealtime() - t_start);
+  }
+}
+
+}  // namespace nmsamd
+
+// ---------------------------------------------------------------------------
+// Launch entry used by the host library (same translation unit keeps the
+// template instantiations private).
+namespace nmsamd {
+
+using KernelFn = void (*)(SearchArgs, int);
+
+template <int ST>
+static KernelFn pick_kernel(int dim, int func) {
+#define NMS_CASE(D)                                                   \
+  switch (func) {                                                     \
+    case kL2Sqr16: return hnsw_search_kernel<D, kL2Sqr16, ST>;        \
+    case kL2Sqr: return hnsw_search_kernel<D, kL2Sqr, ST>;            \
+    case kNormCos: return hnsw_search_kernel<D, kNormCos, ST>;        \
+    case kNegDot: return hnsw_search_kernel<D, kNegDot, ST>;          \
+    default: return nullptr;                                          \
+  }
+  switch (dim) {
+    case 96: NMS_CASE(96)
+    case 100: NMS_CASE(100)
+    case 128: NMS_CASE(128)
+    default: NMS_CASE(0)
+  }
+#undef NMS_CASE
+}
+
+hipError_t launch_search(const SearchArgs& a, int blocks, int waves_per_block, int lds_wave_bytes,
+                         hipStream_t stream) {
+  KernelFn fn = nullptr;
+  if (a.storage == kF32) fn = pick_kernel<kF32>(a.ix.dim, a.ix.func);
+  else if (a.storage == kBF16) fn = pick_kernel<kBF16>(a.ix.dim, a.ix.func);
+  else fn = pick_kernel<kFP8>(a.ix.dim, a.ix.func);
+  if (!fn) return hipErrorInvalidValue;
+  if (waves_per_block < 1 || waves_per_block > kWavesPerBlock) return hipErrorInvalidValue;
+  const size_t lds = (size_t)lds_wave_bytes * waves_per_block;
+  if (lds > 64 * 1024) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(fn, dim3(blocks), dim3(kWave * waves_per_block), lds, stream, a,
+                     lds_wave_bytes);
+  return hipGetLastError();
+}
+
+__global__ void fill_i32_kernel(int* p, int v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill_i32(int* p, int v, int n, hipStream_t stream) {
+  hipLaunchKernelGGL(fill_i32_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, p, v, n);
+  return hipGetLastError();
+}
+
+}  // namespace nmsamd

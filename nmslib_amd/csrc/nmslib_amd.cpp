@@ -1,0 +1,724 @@
+// Host side of the MI355X HNSW query path: the IndexWrapper-like handle behind
+// include/nmslib_amd.h.
+//
+// Reference counterparts (similarity_search/ and python_bindings/ of nmslib):
+//   HnswIndex::load            Hnsw::LoadIndex / LoadOptimizedIndex hnsw.cc:993-1074
+//   HnswIndex::save            Hnsw::SaveIndex / SaveOptimizedIndex hnsw.cc:748-806
+//   HnswIndex::set_query_params Hnsw::SetQueryTimeParams hnsw.cc:472-507
+//   HnswIndex::search          IndexWrapper::knnQueryBatch nmslib.cc:142-184
+//                              + Hnsw::Search dispatch hnsw.cc:717-746
+//   init                       nmslib.init nmslib.cc:470-510, spacefactory.h:46-54
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/nmslib_amd.h"
+#include "hnsw_device.h"
+
+namespace nmsamd {
+hipError_t launch_search(const SearchArgs& a, int blocks, int waves_per_block, int lds_wave_bytes,
+                         hipStream_t stream);
+hipError_t launch_fill_i32(int* p, int v, int n, hipStream_t stream);
+hipError_t launch_convert_lowp(const float* vec, uint16_t* vec16, uint8_t* vec8, float* scale8,
+                               int64_t n, int dim, int dstride, int storage, hipStream_t stream);
+}  // namespace nmsamd
+
+namespace {
+
+using namespace nmsamd;
+
+thread_local std::string g_err;
+
+struct Status : std::runtime_error {
+  int code;
+  Status(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw Status(code, msg); }
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess)
+    fail(NMSAMD_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void ensure(size_t count) {
+    if (count <= n && p) return;
+    release();
+    if (count == 0) return;
+    hip_check(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)), "hipMalloc");
+    n = count;
+  }
+  void upload(const T* src, size_t count, hipStream_t s) {
+    ensure(count);
+    if (count)
+      hip_check(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+  }
+};
+
+enum Algo { kHybrid = 0, kOld = 1, kV1Merge = 2 };
+
+constexpr size_t kLdsPerCu = 160 * 1024;  // gfx950 LDS per CU (and per workgroup)
+
+std::string to_lower(std::string s) {
+  for (auto& c : s) c = (char)std::tolower((unsigned char)c);
+  return s;
+}
+
+template <class T>
+void rd(std::istream& in, T& v) { in.read(reinterpret_cast<char*>(&v), sizeof(T)); }
+template <class T>
+void wr(std::ostream& out, const T& v) { out.write(reinterpret_cast<const char*>(&v), sizeof(T)); }
+
+}  // namespace
+
+struct nmsamd_index {
+  std::string space;
+  int expected_func = 0;  // 0: any l2 variant
+  // query-time parameters (Hnsw defaults after LoadIndex, hnsw.cc:483)
+  size_t ef = 20;
+  Algo algo = kHybrid;
+  int storage = kF32;
+  int rerank_mult = 0;
+  bool collect = false;
+  // file header (hnsw.cc:779-788)
+  bool loaded = false;
+  uint32_t n = 0;
+  uint64_t elem = 0, off_l0 = 0, off_data = 0, maxM = 0, maxM0 = 0, method = 0;
+  int32_t maxlevel = 0, func = 0;
+  uint32_t enter = 0;
+  int dim = 0, dstride = 0, l0stride = 0;
+  // host copies kept for saveIndex (the reference keeps the same block in RAM)
+  std::vector<char> block;
+  std::vector<uint32_t> sizemass;
+  std::vector<int32_t> upper;       // concatenated upper-level blocks
+  std::vector<int32_t> upper_off;   // per node, -1 when none
+  std::vector<std::vector<char>> data_objs;  // load_data objects (.dat)
+  bool has_dups = false;
+  // device copies
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  DevBuf<float> d_vec;
+  DevBuf<uint16_t> d_vec16;
+  DevBuf<uint8_t> d_vec8;
+  DevBuf<float> d_scale8;
+  DevBuf<int32_t> d_l0, d_upoff, d_up, d_ext;
+  DevBuf<uint8_t> d_visited, d_epoch;
+  int64_t vstride = 0;
+  int slots = 0;
+  // per-call buffers
+  DevBuf<float> d_q, d_dists;
+  DevBuf<int32_t> d_ids, d_counts, d_k, d_ef, d_internal, d_work;
+  DevBuf<SearchStats> d_stats;
+  DevBuf<uint32_t> d_qticks;
+  int64_t last_nq = 0;
+  SearchStats last_stats{};
+  double last_ms = 0;
+  std::mutex mu;  // one search at a time per handle
+  int cu_count = 0;
+
+  ~nmsamd_index() {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+void ensure_device(nmsamd_index* h) {
+  if (h->stream) return;
+  int count = 0;
+  hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+  if (count <= 0) fail(NMSAMD_EDEVICE, "no HIP device is visible");
+  hip_check(hipGetDevice(&h->device), "hipGetDevice");
+  hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipEventCreate(&h->ev0), "hipEventCreate");
+  hip_check(hipEventCreate(&h->ev1), "hipEventCreate");
+  hipDeviceProp_t prop;
+  hip_check(hipGetDeviceProperties(&prop, h->device), "hipGetDeviceProperties");
+  h->cu_count = prop.multiProcessorCount;
+}
+
+void parse_and_upload(nmsamd_index* h, const std::string& path) {
+  std::ifstream in(path, std::ios::binary);
+  if (!in) fail(NMSAMD_EIO, "Cannot open file '" + path + "' for reading");
+  in.exceptions(std::ios::badbit | std::ios::failbit);
+  try {
+    uint32_t flag = 0;
+    rd(in, flag);
+    if (!flag)
+      fail(NMSAMD_ERUNTIME,
+           "the regular (non-optimized) HNSW index format is not supported by the device path");
+    rd(in, h->n);
+    rd(in, h->elem);
+    rd(in, h->off_l0);
+    rd(in, h->off_data);
+    rd(in, h->maxlevel);
+    rd(in, h->enter);
+    rd(in, h->maxM);
+    rd(in, h->maxM0);
+    rd(in, h->func);
+    rd(in, h->method);
+    if (h->func < kL2Sqr16 || h->func > 6)
+      fail(NMSAMD_ERUNTIME, "Unknown distance function code: " + std::to_string(h->func));
+    if (h->func > kNegDot)
+      fail(NMSAMD_ERUNTIME, "distance function code " + std::to_string(h->func) +
+                                " (l1/linf) is not supported by the device path");
+    if (h->off_data != 0 || h->off_l0 < 16 || (h->off_l0 - 16) % 4 != 0 ||
+        h->elem != h->off_l0 + 4 * (h->maxM0 + 1))
+      fail(NMSAMD_ERUNTIME, "inconsistent optimized-index header");
+    if (h->maxM0 > (uint64_t)kMaxM0 || h->maxM > (uint64_t)kMaxM0)
+      fail(NMSAMD_ERUNTIME, "maxM0 " + std::to_string(h->maxM0) + " exceeds the device limit of " +
+                                std::to_string(kMaxM0));
+    h->dim = (int)((h->off_l0 - 16) / 4);
+    h->block.resize((size_t)h->n * h->elem);
+    if (!h->block.empty()) in.read(h->block.data(), h->block.size());
+    h->sizemass.assign(h->n, 0);
+    h->upper.clear();
+    h->upper_off.assign(h->n, -1);
+    for (uint32_t i = 0; i < h->n; ++i) {
+      uint32_t sm = 0;
+      rd(in, sm);
+      h->sizemass[i] = sm;
+      if (sm) {
+        if (sm % (4 * (h->maxM + 1)) != 0) fail(NMSAMD_ERUNTIME, "corrupt upper-level list size");
+        h->upper_off[i] = (int32_t)h->upper.size();
+        size_t base = h->upper.size();
+        h->upper.resize(base + sm / 4);
+        in.read(reinterpret_cast<char*>(h->upper.data() + base), sm);
+      }
+    }
+  } catch (const std::ios_base::failure& e) {
+    fail(NMSAMD_EIO, std::string("error reading index '") + path + "': " + e.what());
+  }
+  if (h->n > 0 && h->enter >= h->n) fail(NMSAMD_ERUNTIME, "entry point out of range");
+  const auto elem_node = h->func;  // kept for message clarity
+  (void)elem_node;
+  if (h->expected_func == 1 && h->func != kL2Sqr16 && h->func != kL2Sqr)
+    fail(NMSAMD_ERUNTIME, "index distance function does not match space l2");
+  if (h->expected_func == kNormCos && h->func != kNormCos)
+    fail(NMSAMD_ERUNTIME, "index distance function does not match space cosinesimil");
+  if (h->expected_func == kNegDot && h->func != kNegDot)
+    fail(NMSAMD_ERUNTIME, "index distance function does not match space negdotprod");
+
+  // device layout (hnsw_device.h)
+  const int64_t n = h->n;
+  h->dstride = (h->dim + 3) / 4 * 4;
+  h->l0stride = ((int)h->maxM0 + 31) / 32 * 32;
+  std::vector<float> vec((size_t)std::max<int64_t>(n, 1) * h->dstride, 0.f);
+  std::vector<int32_t> l0((size_t)std::max<int64_t>(n, 1) * h->l0stride, -1);
+  std::vector<int32_t> ext((size_t)std::max<int64_t>(n, 1), 0);
+  bool dups = false;
+  std::vector<int32_t> tmp;
+  for (int64_t i = 0; i < n; ++i) {
+    const char* e = h->block.data() + i * h->elem;
+    std::memcpy(&ext[i], e, 4);
+    std::memcpy(&vec[i * h->dstride], e + 16, 4 * (size_t)h->dim);
+    int32_t cnt;
+    std::memcpy(&cnt, e + h->off_l0, 4);
+    if (cnt < 0 || (uint64_t)cnt > h->maxM0) fail(NMSAMD_ERUNTIME, "corrupt level-0 list count");
+    const int32_t* ids = reinterpret_cast<const int32_t*>(e + h->off_l0 + 4);
+    tmp.assign(ids, ids + cnt);
+    for (int32_t v : tmp)
+      if (v < 0 || v >= n) fail(NMSAMD_ERUNTIME, "level-0 neighbour id out of range");
+    std::memcpy(&l0[i * h->l0stride], ids, 4 * (size_t)cnt);
+    std::sort(tmp.begin(), tmp.end());
+    if (std::adjacent_find(tmp.begin(), tmp.end()) != tmp.end()) dups = true;
+  }
+  for (size_t i = 0; i < h->upper_off.size(); ++i) {
+    if (h->upper_off[i] < 0) continue;
+    const int levels = (int)(h->sizemass[i] / (4 * (h->maxM + 1)));
+    for (int L = 0; L < levels; ++L) {
+      const int32_t* lst = h->upper.data() + h->upper_off[i] + (h->maxM + 1) * L;
+      if (lst[0] < 0 || (uint64_t)lst[0] > h->maxM)
+        fail(NMSAMD_ERUNTIME, "corrupt upper-level list count");
+      for (int j = 1; j <= lst[0]; ++j)
+        if (lst[j] < 0 || lst[j] >= n) fail(NMSAMD_ERUNTIME, "upper-level neighbour id out of range");
+    }
+  }
+  if (n > 0 && h->maxlevel > 0) {
+    const int lev_enter = h->upper_off[h->enter] < 0
+                              ? 0
+                              : (int)(h->sizemass[h->enter] / (4 * (h->maxM + 1)));
+    if (lev_enter < h->maxlevel) fail(NMSAMD_ERUNTIME, "entry point level below maxlevel");
+  }
+  h->has_dups = dups;
+
+  ensure_device(h);
+  hipStream_t s = h->stream;
+  h->d_vec.upload(vec.data(), vec.size(), s);
+  h->d_l0.upload(l0.data(), l0.size(), s);
+  std::vector<int32_t> upoff(h->upper_off.begin(), h->upper_off.end());
+  if (upoff.empty()) upoff.push_back(-1);
+  h->d_upoff.upload(upoff.data(), upoff.size(), s);
+  std::vector<int32_t> up = h->upper;
+  if (up.empty()) up.push_back(0);
+  h->d_up.upload(up.data(), up.size(), s);
+  h->d_ext.upload(ext.data(), ext.size(), s);
+  hip_check(hipStreamSynchronize(s), "upload");
+  h->d_vec16.release();
+  h->d_vec8.release();
+  h->d_scale8.release();
+  h->d_visited.release();
+  h->d_epoch.release();
+  h->slots = 0;
+  h->loaded = true;
+}
+
+// Visited-mark arrays for `slots` resident waves.
+void ensure_slots(nmsamd_index* h, int slots) {
+  if (slots <= h->slots) return;
+  h->vstride = ((int64_t)h->n + 1 + 255) / 256 * 256;
+  h->d_visited.release();
+  h->d_epoch.release();
+  h->d_visited.ensure((size_t)slots * h->vstride);
+  h->d_epoch.ensure((size_t)slots);
+  hip_check(hipMemsetAsync(h->d_visited.p, 0, (size_t)slots * h->vstride, h->stream), "memset");
+  hip_check(hipMemsetAsync(h->d_epoch.p, 0, (size_t)slots, h->stream), "memset");
+  h->slots = slots;
+}
+
+void ensure_lowp(nmsamd_index* h) {
+  if (h->storage == kBF16 && !h->d_vec16.p) {
+    h->d_vec16.ensure((size_t)std::max<int64_t>(h->n, 1) * h->dstride);
+    hip_check(launch_convert_lowp(h->d_vec.p, h->d_vec16.p, nullptr, nullptr, h->n, h->dim,
+                                  h->dstride, kBF16, h->stream), "convert bf16");
+  }
+  if (h->storage == kFP8 && !h->d_vec8.p) {
+    h->d_vec8.ensure((size_t)std::max<int64_t>(h->n, 1) * h->dstride);
+    h->d_scale8.ensure((size_t)std::max<int64_t>(h->n, 1));
+    hip_check(launch_convert_lowp(h->d_vec.p, nullptr, h->d_vec8.p, h->d_scale8.p, h->n, h->dim,
+                                  h->dstride, kFP8, h->stream), "convert fp8");
+  }
+}
+
+DevIndex dev_index(nmsamd_index* h) {
+  DevIndex ix{};
+  ix.vec = h->d_vec.p;
+  ix.vec16 = h->d_vec16.p;
+  ix.vec8 = h->d_vec8.p;
+  ix.scale8 = h->d_scale8.p;
+  ix.l0 = h->d_l0.p;
+  ix.up_off = h->d_upoff.p;
+  ix.up = h->d_up.p;
+  ix.ext_id = h->d_ext.p;
+  ix.visited = h->d_visited.p;
+  ix.epoch = h->d_epoch.p;
+  ix.n = h->n;
+  ix.vstride = h->vstride;
+  ix.dim = h->dim;
+  ix.dstride = h->dstride;
+  ix.l0stride = h->l0stride;
+  ix.maxM = (int)h->maxM;
+  ix.maxM0 = (int)h->maxM0;
+  ix.maxlevel = h->maxlevel;
+  ix.enter = (int)h->enter;
+  ix.func = h->func;
+  ix.has_dups = h->has_dups ? 1 : 0;
+  return ix;
+}
+
+int lds_wave_bytes(int cap, int dstride) {
+  const int b = cap * 8 + dstride * 4 + 64 * 4 * 6;
+  return (b + 15) / 16 * 16;
+}
+
+// Launches the level-0 search of nq queries whose inputs/outputs are device
+// pointers; k_of / ef_of are optional device arrays.
+void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, const int32_t* k_of,
+                const int32_t* ef_of, int max_k, int max_cap, int32_t* ids, float* dists,
+                int32_t* counts, hipStream_t s) {
+  if (!h->loaded) fail(NMSAMD_EINVAL, "Must call createIndex or loadIndex before this method");
+  const bool use_old = h->algo == kOld || (h->algo == kHybrid && max_cap >= 1000 && !ef_of);
+  if (use_old)
+    fail(NMSAMD_ERUNTIME, "SearchOld (ef >= 1000 or algoType=old) is not available on the device");
+  const int cap = std::max(max_cap, 1);
+  const int lds = lds_wave_bytes(cap, h->dstride);
+  if ((size_t)lds > kLdsPerCu)
+    fail(NMSAMD_EINVAL, "max(ef, k) = " + std::to_string(cap) + " exceeds the LDS budget of one wave");
+  ensure_lowp(h);
+  // waves per block and resident blocks per CU from the LDS footprint; the
+  // grid is persistent: as many resident waves as the device holds, at most
+  // one per query
+  const int wpb = (int)std::max<size_t>(1, std::min<size_t>(kWavesPerBlock, kLdsPerCu / lds));
+  const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, kLdsPerCu / ((size_t)lds * wpb)));
+  int64_t max_blocks = (int64_t)h->cu_count * per_cu;
+  // visited marks take n bytes per slot: keep them under ~16 GiB
+  const int64_t vis_budget = (int64_t)16 << 30;
+  const int64_t slot_cap = std::max<int64_t>(wpb, vis_budget / std::max<int64_t>(h->n, 1));
+  max_blocks = std::max<int64_t>(1, std::min<int64_t>(max_blocks, slot_cap / wpb));
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((nq + wpb - 1) / wpb, max_blocks));
+  ensure_slots(h, blocks * wpb);
+  h->d_work.ensure(1);
+  hip_check(launch_fill_i32(h->d_work.p, 0, 1, s), "fill");
+  if (h->collect) {
+    h->d_stats.ensure(1);
+    hip_check(launch_fill_i32(reinterpret_cast<int*>(h->d_stats.p), 0,
+                              sizeof(SearchStats) / sizeof(int), s), "fill");
+  }
+  SearchArgs a{};
+  a.ix = dev_index(h);
+  a.queries = d_q;
+  a.nq = nq;
+  a.k_of = k_of;
+  a.ef_of = ef_of;
+  a.k = k;
+  a.ef = ef;
+  a.cap = cap;
+  a.out_stride = max_k;
+  a.out_ids = ids;
+  a.out_dists = dists;
+  a.out_counts = counts;
+  a.out_internal = nullptr;
+  a.work = h->d_work.p;
+  a.stats = h->collect ? h->d_stats.p : nullptr;
+  if (h->collect) h->d_qticks.ensure((size_t)nq);
+  a.q_ticks = h->collect ? h->d_qticks.p : nullptr;
+  h->last_nq = nq;
+  a.storage = h->storage;
+  hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
+  hip_check(launch_search(a, blocks, wpb, lds, s), "search kernel launch");
+  hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
+}
+
+void finish_stats(nmsamd_index* h) {
+  float ms = 0;
+  hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
+  h->last_ms = ms;
+  if (h->collect)
+    hip_check(hipMemcpy(&h->last_stats, h->d_stats.p, sizeof(SearchStats), hipMemcpyDeviceToHost),
+              "stats");
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return NMSAMD_OK;
+  } catch (const Status& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::invalid_argument& e) {
+    g_err = e.what();
+    return NMSAMD_EINVAL;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return NMSAMD_ERUNTIME;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* nmsamd_last_error(void) { return g_err.c_str(); }
+
+int nmsamd_init(const char* space, const char* method, nmsamd_index** out) {
+  return guarded([&] {
+    if (!out) fail(NMSAMD_EINVAL, "null output handle");
+    const std::string sp = space ? space : "cosinesimil";
+    const std::string me = method ? method : "hnsw";
+    auto h = std::make_unique<nmsamd_index>();
+    if (sp == "l2") h->expected_func = 1;
+    else if (sp == "cosinesimil") h->expected_func = kNormCos;
+    else if (sp == "negdotprod") h->expected_func = kNegDot;
+    else fail(NMSAMD_ERUNTIME, "It looks like the space " + sp +
+                                   " is not defined for the distance type : FLOAT");
+    if (me != "hnsw")
+      fail(NMSAMD_ERUNTIME, "It looks like the method " + me +
+                                " is not defined for the distance type : FLOAT (only hnsw is available)");
+    h->space = sp;
+    *out = h.release();
+  });
+}
+
+int nmsamd_free(nmsamd_index* h) {
+  return guarded([&] { delete h; });
+}
+
+int nmsamd_load_index(nmsamd_index* h, const char* filename, int load_data) {
+  return guarded([&] {
+    if (!h || !filename) fail(NMSAMD_EINVAL, "null argument");
+    std::lock_guard<std::mutex> g(h->mu);
+    h->loaded = false;
+    h->data_objs.clear();
+    if (load_data) {
+      const std::string dat = std::string(filename) + ".dat";
+      std::ifstream in(dat, std::ios::binary);
+      if (!in) fail(NMSAMD_EIO, "Cannot open file '" + dat + "' for reading");
+      in.exceptions(std::ios::badbit | std::ios::failbit);
+      try {
+        uint64_t qty = 0;
+        rd(in, qty);
+        for (uint64_t i = 0; i < qty; ++i) {
+          uint64_t sz = 0;
+          rd(in, sz);
+          std::vector<char> obj(sz);
+          in.read(obj.data(), sz);
+          h->data_objs.push_back(std::move(obj));
+        }
+      } catch (const std::ios_base::failure& e) {
+        fail(NMSAMD_EIO, "error reading '" + dat + "': " + e.what());
+      }
+    }
+    parse_and_upload(h, filename);
+    h->ef = 20;  // ResetQueryTimeParams (nmslib.cc:110-112)
+    h->algo = kHybrid;
+  });
+}
+
+int nmsamd_save_index(nmsamd_index* h, const char* filename, int save_data) {
+  return guarded([&] {
+    if (!h || !filename) fail(NMSAMD_EINVAL, "null argument");
+    if (!h->loaded) fail(NMSAMD_EINVAL, "Must call createIndex or loadIndex before this method");
+    if (save_data) {
+      const std::string dat = std::string(filename) + ".dat";
+      std::ofstream out(dat, std::ios::binary);
+      if (!out) fail(NMSAMD_EIO, "Cannot open file '" + dat + "' for writing");
+      wr(out, (uint64_t)h->data_objs.size());
+      for (auto& o : h->data_objs) {
+        wr(out, (uint64_t)o.size());
+        out.write(o.data(), o.size());
+      }
+    }
+    std::ofstream out(filename, std::ios::binary);
+    if (!out) fail(NMSAMD_EIO, std::string("Cannot open file '") + filename + "' for writing");
+    wr(out, (uint32_t)1);
+    wr(out, h->n);
+    wr(out, h->elem);
+    wr(out, h->off_l0);
+    wr(out, h->off_data);
+    wr(out, h->maxlevel);
+    wr(out, h->enter);
+    wr(out, h->maxM);
+    wr(out, h->maxM0);
+    wr(out, h->func);
+    wr(out, h->method);
+    out.write(h->block.data(), h->block.size());
+    for (uint32_t i = 0; i < h->n; ++i) {
+      wr(out, h->sizemass[i]);
+      if (h->sizemass[i])
+        out.write(reinterpret_cast<const char*>(h->upper.data() + h->upper_off[i]), h->sizemass[i]);
+    }
+    if (!out) fail(NMSAMD_EIO, std::string("error writing '") + filename + "'");
+  });
+}
+
+int nmsamd_set_query_time_params(nmsamd_index* h, const char* const* kv, int n) {
+  return guarded([&] {
+    if (!h) fail(NMSAMD_EINVAL, "null handle");
+    std::vector<std::pair<std::string, std::string>> params;
+    for (int i = 0; i < n; ++i) {
+      const std::string s = kv[i] ? kv[i] : "";
+      const size_t eq = s.find('=');
+      if (eq == std::string::npos || s.find('=', eq + 1) != std::string::npos)
+        fail(NMSAMD_ERUNTIME, "Wrong format of an argument: '" + s +
+                                  "' should be in the format: <Name>=<Value>");
+      const std::string name = s.substr(0, eq);
+      for (auto& p : params)
+        if (p.first == name) fail(NMSAMD_ERUNTIME, "Duplicate parameter: " + name);
+      params.emplace_back(name, s.substr(eq + 1));
+    }
+    auto has = [&](const char* name) {
+      for (auto& p : params) if (p.first == name) return true;
+      return false;
+    };
+    if (has("ef") && has("efSearch"))
+      fail(NMSAMD_ERUNTIME,
+           "The user shouldn't specify parameters ef and efSearch at the same time (they are synonyms)");
+    auto to_size = [](const std::string& v) {
+      std::stringstream ss(v);
+      size_t out = 0;
+      if (!(ss >> out) || !ss.eof() || v.find('-') != std::string::npos)
+        fail(NMSAMD_ERUNTIME, "Failed to convert value '" + v + "' from type: m");
+      return out;
+    };
+    size_t ef = 20;
+    Algo algo = kHybrid;
+    for (auto& p : params) {
+      if (p.first == "ef" || p.first == "efSearch") {
+        ef = to_size(p.second);
+      } else if (p.first == "searchMethod") {
+        std::stringstream ss(p.second);
+        int tmp;
+        if (!(ss >> tmp) || !ss.eof())
+          fail(NMSAMD_ERUNTIME, "Failed to convert value '" + p.second + "' from type: i");
+      } else if (p.first == "algoType") {
+        const std::string t = to_lower(p.second);
+        if (t == "v1merge") algo = kV1Merge;
+        else if (t == "old") algo = kOld;
+        else if (t == "hybrid") algo = kHybrid;
+        else fail(NMSAMD_ERUNTIME, "algoType should be one of the following: old, v1merge");
+      } else {
+        fail(NMSAMD_ERUNTIME, "Unknown parameters found!");
+      }
+    }
+    h->ef = ef;
+    h->algo = algo;
+  });
+}
+
+int nmsamd_knn_query_batch(nmsamd_index* h, const float* q, int64_t nq, int64_t dim, int32_t k,
+                           int32_t num_threads, int32_t* ids, float* dists, int32_t* counts) {
+  (void)num_threads;
+  return guarded([&] {
+    if (!h) fail(NMSAMD_EINVAL, "null handle");
+    std::lock_guard<std::mutex> g(h->mu);
+    if (!h->loaded) fail(NMSAMD_EINVAL, "Must call createIndex or loadIndex before this method");
+    if (nq < 0 || k < 0) fail(NMSAMD_EINVAL, "nq and k must be non-negative");
+    if (dim != h->dim)
+      fail(NMSAMD_EINVAL, "query dimension " + std::to_string(dim) + " != index dimension " +
+                              std::to_string(h->dim));
+    if (nq == 0) return;
+    if (k == 0) {
+      std::fill(counts, counts + nq, 0);
+      return;
+    }
+    hipStream_t s = h->stream;
+    h->d_q.upload(q, (size_t)nq * dim, s);
+    h->d_ids.ensure((size_t)nq * k);
+    h->d_dists.ensure((size_t)nq * k);
+    h->d_counts.ensure((size_t)nq);
+    const int cap = (int)std::max<size_t>(h->ef, (size_t)k);
+    run_search(h, h->d_q.p, nq, k, (int)h->ef, nullptr, nullptr, k, cap, h->d_ids.p,
+               h->d_dists.p, h->d_counts.p, s);
+    hip_check(hipMemcpyAsync(ids, h->d_ids.p, sizeof(int32_t) * nq * k, hipMemcpyDeviceToHost, s), "D2H");
+    hip_check(hipMemcpyAsync(dists, h->d_dists.p, sizeof(float) * nq * k, hipMemcpyDeviceToHost, s), "D2H");
+    hip_check(hipMemcpyAsync(counts, h->d_counts.p, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, s), "D2H");
+    hip_check(hipStreamSynchronize(s), "search");
+    finish_stats(h);
+  });
+}
+
+int nmsamd_knn_query_batch_device(nmsamd_index* h, const float* d_q, int64_t nq, int32_t k,
+                                  int32_t* d_ids, float* d_dists, int32_t* d_counts, void* stream) {
+  return guarded([&] {
+    if (!h) fail(NMSAMD_EINVAL, "null handle");
+    std::lock_guard<std::mutex> g(h->mu);
+    if (!h->loaded) fail(NMSAMD_EINVAL, "Must call createIndex or loadIndex before this method");
+    if (nq < 0 || k <= 0) fail(NMSAMD_EINVAL, "nq must be non-negative and k positive");
+    if (nq == 0) return;
+    hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : h->stream;
+    const int cap = (int)std::max<size_t>(h->ef, (size_t)k);
+    run_search(h, d_q, nq, k, (int)h->ef, nullptr, nullptr, k, cap, d_ids, d_dists, d_counts, s);
+  });
+}
+
+int nmsamd_knn_query_batch_uneven(nmsamd_index* h, const float* q, int64_t nq, int64_t dim,
+                                  const int32_t* k, const int32_t* ef, int32_t max_k,
+                                  int32_t* ids, float* dists, int32_t* counts) {
+  return guarded([&] {
+    if (!h) fail(NMSAMD_EINVAL, "null handle");
+    std::lock_guard<std::mutex> g(h->mu);
+    if (!h->loaded) fail(NMSAMD_EINVAL, "Must call createIndex or loadIndex before this method");
+    if (dim != h->dim) fail(NMSAMD_EINVAL, "query dimension does not match the index");
+    if (nq <= 0) return;
+    int cap = 1, mk = 0;
+    for (int64_t i = 0; i < nq; ++i) {
+      if (k[i] < 0 || ef[i] < 0) fail(NMSAMD_EINVAL, "k and ef must be non-negative");
+      cap = std::max(cap, std::max(k[i], ef[i]));
+      mk = std::max(mk, k[i]);
+    }
+    if (mk > max_k) fail(NMSAMD_EINVAL, "max_k is smaller than a query's k");
+    if (max_k <= 0) {
+      std::fill(counts, counts + nq, 0);
+      return;
+    }
+    hipStream_t s = h->stream;
+    h->d_q.upload(q, (size_t)nq * dim, s);
+    h->d_k.upload(k, (size_t)nq, s);
+    h->d_ef.upload(ef, (size_t)nq, s);
+    h->d_ids.ensure((size_t)nq * max_k);
+    h->d_dists.ensure((size_t)nq * max_k);
+    h->d_counts.ensure((size_t)nq);
+    run_search(h, h->d_q.p, nq, 0, 0, h->d_k.p, h->d_ef.p, max_k, cap, h->d_ids.p, h->d_dists.p,
+               h->d_counts.p, s);
+    hip_check(hipMemcpyAsync(ids, h->d_ids.p, sizeof(int32_t) * nq * max_k, hipMemcpyDeviceToHost, s), "D2H");
+    hip_check(hipMemcpyAsync(dists, h->d_dists.p, sizeof(float) * nq * max_k, hipMemcpyDeviceToHost, s), "D2H");
+    hip_check(hipMemcpyAsync(counts, h->d_counts.p, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, s), "D2H");
+    hip_check(hipStreamSynchronize(s), "search");
+    finish_stats(h);
+  });
+}
+
+int64_t nmsamd_size(const nmsamd_index* h) { return h ? (int64_t)h->data_objs.size() : 0; }
+
+int nmsamd_index_info(const nmsamd_index* h, int64_t* o) {
+  return guarded([&] {
+    if (!h || !o) fail(NMSAMD_EINVAL, "null argument");
+    o[0] = h->n; o[1] = h->dim; o[2] = (int64_t)h->maxM; o[3] = (int64_t)h->maxM0;
+    o[4] = h->maxlevel; o[5] = h->enter; o[6] = h->func; o[7] = (int64_t)h->ef;
+  });
+}
+
+int nmsamd_set_precision(nmsamd_index* h, int storage, int rerank_mult) {
+  return guarded([&] {
+    if (!h) fail(NMSAMD_EINVAL, "null handle");
+    if (storage < kF32 || storage > kFP8) fail(NMSAMD_EINVAL, "storage must be 0 (fp32), 1 (bf16) or 2 (fp8)");
+    if (rerank_mult < 0) fail(NMSAMD_EINVAL, "rerank_mult must be >= 0");
+    h->storage = storage;
+    h->rerank_mult = rerank_mult;
+  });
+}
+
+int nmsamd_last_stats(const nmsamd_index* hc, uint64_t* o, double* ms) {
+  return guarded([&] {
+    if (!hc) fail(NMSAMD_EINVAL, "null handle");
+    // the launch may have been asynchronous (device entry point): read the
+    // events and counters now, after the caller synchronised
+    auto* h = const_cast<nmsamd_index*>(hc);
+    if (h->ev1 && hipEventQuery(h->ev1) == hipSuccess) finish_stats(h);
+    if (o) {
+      o[0] = h->last_stats.expansions;
+      o[1] = h->last_stats.dist_level0;
+      o[2] = h->last_stats.dist_upper;
+      o[3] = h->last_stats.list_ints_read;
+      o[4] = h->last_stats.tie_paths;
+    }
+    if (ms) *ms = h->last_ms;
+  });
+}
+
+int nmsamd_last_query_us(const nmsamd_index* h, float* out_us, int64_t nq) {
+  return guarded([&] {
+    if (!h || !out_us) fail(NMSAMD_EINVAL, "null argument");
+    if (!h->collect || !h->d_qticks.p) fail(NMSAMD_EINVAL, "statistics collection is off");
+    if (nq > h->last_nq) fail(NMSAMD_EINVAL, "more queries requested than the last search held");
+    std::vector<uint32_t> t((size_t)nq);
+    hip_check(hipMemcpy(t.data(), h->d_qticks.p, sizeof(uint32_t) * nq, hipMemcpyDeviceToHost),
+              "query times");
+    for (int64_t i = 0; i < nq; ++i) out_us[i] = (float)(t[i] * 0.01);  // 100 MHz clock
+  });
+}
+
+int nmsamd_collect_stats(nmsamd_index* h, int on) {
+  return guarded([&] {
+    if (!h) fail(NMSAMD_EINVAL, "null handle");
+    h->collect = on != 0;
+  });
+}
+
+}  // extern "C"

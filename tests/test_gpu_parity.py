@@ -1,0 +1,138 @@
+"""Parity of the HIP search with the reference (golden fixtures written from
+the compiled reference) and with the CPU oracle, through the C ABI.
+
+Bar: ids and FP32 distances bit-exact, with exactly tied distances compared
+as sets (the reference orders them by heap address, knnqueue.h:73-74)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import fixtures as F
+from tests.gpu_util import load
+
+pytestmark = pytest.mark.gpu
+
+V1_RUNS = {name: [r for r in F.runs(name) if r[2] != "old" and r[0] < 1000] for name in F.NAMES}
+
+
+def assert_same(ids, dists, counts, rids, rdists, rcounts, msg):
+    np.testing.assert_array_equal(counts, rcounts, err_msg=msg)
+    ci, cd = F.canon(ids, dists)
+    ri, rd = F.canon(rids, rdists)
+    np.testing.assert_array_equal(ci, ri, err_msg=msg)
+    np.testing.assert_array_equal(cd.view(np.uint32), rd.view(np.uint32), err_msg=msg)
+
+
+@pytest.mark.parametrize("name", F.NAMES)
+def test_gpu_matches_reference_goldens(name):
+    m = F.META[name]
+    idx = load(name, F.index_path(name), m["space"])
+    res = F.results(name)
+    q = res["queries"]
+    for ef, k, algo in V1_RUNS[name]:
+        idx.setQueryTimeParams({"efSearch": ef, "algoType": algo})
+        ids, dists, counts = idx.knnQueryBatchArrays(q, k)
+        t = F.tag(ef, k, algo)
+        print(f"{name} {t}", flush=True)
+        assert_same(ids, dists, counts, res[t + "_ids"], res[t + "_dists"], res[t + "_counts"],
+                    f"{name} {t}")
+
+
+@pytest.mark.parametrize("name", ["u128_l2", "hn128_l2", "dec100_cos", "gmm96_nd"])
+@pytest.mark.parametrize("ef,k", [(24, 10), (48, 5), (200, 50), (300, 1)])
+def test_gpu_matches_oracle(name, ef, k):
+    m = F.META[name]
+    idx = load(name, F.index_path(name), m["space"], ef)
+    q = F.results(name)["queries"]
+    ids, dists, counts = idx.knnQueryBatchArrays(q, k)
+    oi, od, oc = O.OracleIndex(F.index_path(name)).search(q, k, ef)
+    np.testing.assert_array_equal(ids, oi)  # the oracle orders ties by internal index too
+    np.testing.assert_array_equal(dists.view(np.uint32), od.view(np.uint32))
+    np.testing.assert_array_equal(counts, oc)
+
+
+def test_gpu_config1_full_batch():
+    # 10,000 x 128 uniform, l2, M=16, efC=200; 1,000 queries, k=10, ef=100
+    name = "cfg1_l2"
+    idx = load(name, F.index_path(name), "l2", 100)
+    res = F.results(name)
+    idx.collectStats(True)
+    ids, dists, counts = idx.knnQueryBatchArrays(res["queries"], 10)
+    assert_same(ids, dists, counts, res["ef100_k10_hybrid_ids"], res["ef100_k10_hybrid_dists"],
+                res["ef100_k10_hybrid_counts"], "cfg1")
+    st = idx.lastStats()
+    _, _, _, ost = O.OracleIndex(F.index_path(name)).search(res["queries"], 10, 100,
+                                                            with_stats=True)
+    assert st["expansions"] == ost["expansions"]
+    assert st["dist_level0"] == ost["dist_level0"]
+    assert st["dist_upper"] == ost["dist_upper"]
+    assert st["list_ints_read"] == ost["list_ints_read"]
+
+
+def test_gpu_uneven_batch_equals_single_runs():
+    name = "u128_l2"
+    idx = load(name, F.index_path(name), "l2")
+    q = F.results(name)["queries"]
+    rng = np.random.default_rng(0)
+    ks = rng.choice([1, 10, 100], size=len(q)).astype(np.int32)
+    efs = rng.integers(16, 400, size=len(q)).astype(np.int32)
+    ids, dists, counts = idx.knnQueryBatchUneven(q, ks, efs)
+    oracle = O.OracleIndex(F.index_path(name))
+    for i in range(len(q)):
+        oi, od, oc = oracle.search(q[i:i + 1], int(ks[i]), int(efs[i]))
+        assert counts[i] == oc[0]
+        np.testing.assert_array_equal(ids[i, :oc[0]], oi[0, :oc[0]])
+        np.testing.assert_array_equal(dists[i, :oc[0]], od[0, :oc[0]])
+
+
+def test_gpu_edge_cases():
+    name = "tiny_l2"
+    idx = load(name, F.index_path(name), "l2", 10)
+    q = F.results(name)["queries"]
+    # k larger than the index: every element comes back once
+    ids, dists, counts = idx.knnQueryBatchArrays(q[:3], 100)
+    assert np.all(counts == 60)
+    assert np.all(ids[:, 60:] == -1)
+    for row in ids:
+        assert sorted(row[:60].tolist()) == list(range(60))
+    # k = 0 and an empty batch
+    ids, dists, counts = idx.knnQueryBatchArrays(q[:3], 0)
+    assert counts.tolist() == [0, 0, 0]
+    assert idx.knnQueryBatch(np.zeros((0, 128), np.float32), 5) == []
+    # knnQuery is a batch of one, list-of-vectors input and Fortran order agree
+    one = idx.knnQuery(q[0], k=5)
+    lst = idx.knnQueryBatch([list(v) for v in q[:2]], k=5)
+    fort = idx.knnQueryBatch(np.asfortranarray(q[:2]), k=5)
+    np.testing.assert_array_equal(one[0], lst[0][0])
+    np.testing.assert_array_equal(lst[1][0], fort[1][0])
+    with pytest.raises(ValueError):
+        idx.knnQueryBatch(np.zeros((2, 7), np.float32), 5)
+
+
+def test_gpu_zero_cosine_query_gives_distance_one():
+    # bindings_test.py:73-91: an all-zero query on cosinesimil returns distance 1
+    name = "dec100_cos"
+    idx = load(name, F.index_path(name), "cosinesimil", 10)
+    ids, dists = idx.knnQuery(np.zeros(100, np.float32), k=3)
+    assert dists.tolist() == [1.0, 1.0, 1.0]
+
+
+def test_gpu_save_index_round_trip(tmp_path):
+    name = "gmm96_nd"
+    idx = load(name, F.index_path(name), "negdotprod")
+    out = tmp_path / "copy.hnsw"
+    idx.saveIndex(str(out))
+    assert open(out, "rb").read() == open(F.index_path(name), "rb").read()
+    idx2 = load(name, str(out), "negdotprod", 64)
+    idx.setQueryTimeParams({"efSearch": 64})
+    q = F.results(name)["queries"][:20]
+    a = idx.knnQueryBatchArrays(q, 10)
+    b = idx2.knnQueryBatchArrays(q, 10)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_gpu_wrong_space_for_index_is_rejected():
+    name = "gmm96_nd"
+    with pytest.raises(RuntimeError, match="does not match space l2"):
+        load(name, F.index_path(name), "l2")

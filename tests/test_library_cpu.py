@@ -1,0 +1,77 @@
+"""CPU-side checks of the product boundary: the HIP library loads, exports
+every entry point include/nmslib_amd.h declares, and the host logic that runs
+without a device (init, parameter parsing, error mapping) behaves as the
+reference binding does."""
+import os
+import re
+
+import pytest
+
+import nmslib_amd as nmslib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "nmslib_amd.h")).read()
+    return sorted(set(re.findall(r"\b(nmsamd_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = nmslib.load_library()
+    syms = header_symbols()
+    assert len(syms) >= 14
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(nmslib.SIGNATURES)
+
+
+def test_init_spaces_and_methods():
+    for sp in ("l2", "cosinesimil", "negdotprod"):
+        idx = nmslib.init(method="hnsw", space=sp)
+        assert "hnsw" in repr(idx)
+    with pytest.raises(RuntimeError, match="space foo is not defined"):
+        nmslib.init(space="foo")
+    with pytest.raises(RuntimeError, match="method sw-graph"):
+        nmslib.init(space="l2", method="sw-graph")
+    with pytest.raises(ValueError):
+        nmslib.init(space="l2", data_type=nmslib.DataType.SPARSE_VECTOR)
+
+
+def test_query_time_params_semantics():
+    # Hnsw::SetQueryTimeParams (hnsw.cc:472-507)
+    idx = nmslib.init(space="l2")
+    idx.setQueryTimeParams({"efSearch": 77})
+    assert idx.info()["ef"] == 77
+    idx.setQueryTimeParams(["ef=33", "algoType=v1merge"])
+    assert idx.info()["ef"] == 33
+    idx.setQueryTimeParams()  # None resets to the default ef = 20
+    assert idx.info()["ef"] == 20
+    with pytest.raises(RuntimeError, match="synonyms"):
+        idx.setQueryTimeParams({"ef": 10, "efSearch": 10})
+    with pytest.raises(RuntimeError, match="Unknown parameters found!"):
+        idx.setQueryTimeParams({"efConstruction": 10})
+    with pytest.raises(RuntimeError, match="algoType should be one of"):
+        idx.setQueryTimeParams({"algoType": "fast"})
+    with pytest.raises(RuntimeError, match="Failed to convert value 'x'"):
+        idx.setQueryTimeParams({"ef": "x"})
+    with pytest.raises(RuntimeError, match="Wrong format"):
+        idx.setQueryTimeParams(["ef"])
+    with pytest.raises(ValueError, match="Unknown type for parameter"):
+        idx.setQueryTimeParams({"ef": [1]})
+    idx.setQueryTimeParams({"searchMethod": 3, "algoType": "OLD"})
+
+
+def test_search_before_load_is_value_error():
+    idx = nmslib.init(space="l2")
+    with pytest.raises(ValueError, match="Must call createIndex or loadIndex"):
+        idx.knnQueryBatch([[0.0] * 4], k=1)
+    with pytest.raises(ValueError, match="Must call createIndex or loadIndex"):
+        idx.saveIndex("/nonexistent/x")
+    assert len(idx) == 0
+
+
+def test_load_missing_file_is_runtime_error(tmp_path):
+    idx = nmslib.init(space="l2")
+    with pytest.raises(RuntimeError, match="Cannot open file"):
+        idx.loadIndex(str(tmp_path / "missing.bin"))
