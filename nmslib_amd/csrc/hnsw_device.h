@@ -79,11 +79,16 @@ struct SearchArgs {
   int* work;                 // work counter, zero before the launch
   SearchStats* stats;        // optional
   int storage;               // Storage
-  int algo_old;              // 1: SearchOld semantics
-  float* old_heap_key;       // SearchOld scratch per slot
+  int rerank;                // > 0: exact FP32 rerank of the best rerank*k candidates
+  int algo;                  // AlgoMode
+  float* old_heap_key;       // SearchOld candidate heap, old_heap_cap per slot
   int32_t* old_heap_id;
   int64_t old_heap_cap;
+  int* err;                  // set to 1 when a SearchOld candidate heap overflows
 };
+
+// algoType (hnsw.cc:491-501); hybrid picks SearchOld for ef >= 1000 per query
+enum AlgoMode : int { kAlgoHybrid = 0, kAlgoOld = 1, kAlgoV1Merge = 2 };
 
 }  // namespace nmsamd
 #endif

@@ -547,6 +547,53 @@ __device__ void v1merge_query(const SearchArgs& a, const WaveLds& w, uint8_t* vi
   }
   WSYNC();
 
+  if constexpr (ST != kF32) {
+    // exact FP32 rerank of the best R candidates of the reduced-precision
+    // traversal: distances recomputed from the FP32 vectors in the reference's
+    // summation order, then the R items re-sorted by (distance, internal id)
+    if (a.rerank > 0) {
+      const int R = min(n, max(k, a.rerank * k));
+      for (int b = 0; b < R; b += 32) {
+        const int s = b + (lane >> 1);
+        const bool ok = s < R;
+        const int id = ok ? (int)(w.ids[s] & ~kUsedBit) : 0;
+        const float dd = dist_pair_f32<D, FUNC>(w.qs, ix.vec + (int64_t)id * ix.dstride, h, ix.dim);
+        if (ok && h == 0) {
+          w.keys[s] = dd;
+          w.ids[s] = (uint32_t)id;
+        }
+      }
+      WSYNC();
+      // rank sort in place through registers, 64 items per round
+      for (int b = 0; b < R; b += 64) {
+        const int p = b + lane;
+        float key = 0.f;
+        uint32_t id = 0;
+        int rank = 0;
+        if (p < R) {
+          key = w.keys[p];
+          id = w.ids[p];
+          for (int j = 0; j < R; ++j) {
+            const float kj = w.keys[j];
+            rank += (kj < key) || (kj == key && w.ids[j] < id);
+          }
+        }
+        if (p < R) {
+          // keys[R, 2R) is free (the capacity holds R more items): stage there
+          w.keys[R + rank] = key;
+          w.ids[R + rank] = id;
+        }
+        WSYNC();
+      }
+      for (int p = lane; p < R; p += 64) {
+        w.keys[p] = w.keys[R + p];
+        w.ids[p] = w.ids[R + p];
+      }
+      WSYNC();
+      n = R;
+    }
+  }
+
   // report the first min(k, n) items, ties by internal index
   const int kk = min(k, n);
   const int64_t row = (int64_t)qi * a.out_stride;
@@ -576,6 +623,152 @@ __device__ void v1merge_query(const SearchArgs& a, const WaveLds& w, uint8_t* vi
       atomicAdd(&a.stats->dist_upper, ndu);
       atomicAdd(&a.stats->list_ints_read, nl);
       atomicAdd(&a.stats->tie_paths, nties);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SearchOld at level 0 (hnsw_distfunc_opt.cc:99-148): a candidate heap
+// ordered by -distance (unbounded, in this slot's HBM scratch), a result heap
+// of at most ef items and the k-item result queue.  Admission of each fresh
+// neighbour depends on the heap state the previous one left, so the wave
+// computes the distances of an expansion in parallel and lane 0 replays the
+// admissions in list order with std::priority_queue's heap steps.  The LDS
+// sorted-array space holds the ef-heap in [0, ef] and the result queue
+// (sorted by (distance, internal id), the order KNNQueue pops in) after it.
+template <int D, int FUNC, int ST>
+__device__ void old_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, uint8_t ep, int qi,
+                          int k, int ef, int lane, int h, uint64_t ltm, int64_t slot) {
+  const DevIndex& ix = a.ix;
+  unsigned long long nexp = 0, nd0 = 0, ndu = 1, nl = 0;
+  float* hk = a.old_heap_key + slot * a.old_heap_cap;   // candidate heap keys (-d)
+  int32_t* hi = a.old_heap_id + slot * a.old_heap_cap;
+  float* bk = w.keys;                                   // ef-heap (max on d)
+  int32_t* bi = reinterpret_cast<int32_t*>(w.ids);
+  float* rk = w.keys + (ef + 1);                        // result queue, ascending
+  int32_t* ri = reinterpret_cast<int32_t*>(w.ids) + (ef + 1);
+
+  int cur = ix.enter;
+  float curd = dist_node<D, FUNC, ST>(w.qs, ix, cur, h);
+  greedy_descent<D, FUNC, ST>(ix, w.qs, lane, h, cur, curd, ndu, nl);
+
+  // lane 0 owns the heap sizes; they are broadcast when the wave needs them
+  int hs = 0, bs = 0, rs = 0;
+  bool overflow = false;
+  auto offer_result = [&](float d, int id) {  // KNNQuery::CheckAndAddToResult
+    if (rs < k || d < rk[rs - 1]) {
+      int p = rs < k ? rs : k - 1;  // slot freed by the popped maximum
+      if (rs < k) ++rs;
+      while (p > 0 && (rk[p - 1] > d || (rk[p - 1] == d && ri[p - 1] > id))) {
+        rk[p] = rk[p - 1];
+        ri[p] = ri[p - 1];
+        --p;
+      }
+      rk[p] = d;
+      ri[p] = id;
+    }
+  };
+  if (lane == 0) {
+    hk[0] = -curd;
+    hi[0] = cur;
+    hs = 1;
+    bk[0] = curd;
+    bi[0] = cur;
+    bs = 1;
+    offer_result(curd, cur);
+    vis[cur] = ep;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  for (;;) {
+    int node = -1;
+    if (lane == 0) {
+      if (hs > 0 && !(-hk[0] > bk[0])) {
+        node = hi[0];
+        emu_heap_pop(hk, hi, hs);
+        --hs;
+      }
+    }
+    node = __shfl(node, 0);
+    if (node < 0) break;
+    ++nexp;
+    const int32_t* lst = ix.l0 + (int64_t)node * ix.l0stride;
+    const int t = lane < ix.maxM0 ? lst[lane] : -1;
+    const bool valid = t >= 0;
+    nl += 1 + __popcll(__ballot(valid));
+    bool fresh = false;
+    if (valid) fresh = vis[t] != ep;
+    if (ix.has_dups) {
+      w.cand[lane] = t;
+      WSYNC();
+      for (int j = 0; j < lane; ++j)
+        if (w.cand[j] == t) fresh = false;
+      WSYNC();
+    }
+    if (fresh) vis[t] = ep;
+    const uint64_t fm = __ballot(fresh);
+    const int ncand = __popcll(fm);
+    if (fresh) w.cand[__popcll(fm & ltm)] = t;
+    nd0 += ncand;
+    WSYNC();
+    for (int b = 0; b < ncand; b += 32) {
+      const int s = b + (lane >> 1);
+      const bool ok = s < ncand;
+      const int id = ok ? w.cand[s] : node;
+      const float dd = dist_node<D, FUNC, ST>(w.qs, ix, id, h);
+      if (ok && h == 0) w.cdist[s] = dd;
+    }
+    WSYNC();
+    if (lane == 0) {
+      for (int j = 0; j < ncand; ++j) {
+        const float d = w.cdist[j];
+        const int id = w.cand[j];
+        if (bk[0] > d || bs < ef) {
+          if (hs >= a.old_heap_cap) {
+            overflow = true;
+            break;
+          }
+          hk[hs] = -d;
+          hi[hs] = id;
+          ++hs;
+          emu_heap_push(hk, hi, hs);
+          offer_result(d, id);
+          bk[bs] = d;
+          bi[bs] = id;
+          ++bs;
+          emu_heap_push(bk, bi, bs);
+          if (bs > ef) {
+            emu_heap_pop(bk, bi, bs);
+            --bs;
+          }
+        }
+      }
+    }
+    if (__shfl((int)overflow, 0)) break;
+    WSYNC();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  }
+  WSYNC();
+  rs = __shfl(rs, 0);
+  if (__shfl((int)overflow, 0) && lane == 0 && a.err) atomicOr(a.err, 1);
+  const int64_t row = (int64_t)qi * a.out_stride;
+  for (int p = lane; p < k; p += 64) {
+    if (p < rs) {
+      a.out_ids[row + p] = ix.ext_id[ri[p]];
+      a.out_dists[row + p] = rk[p];
+      if (a.out_internal) a.out_internal[row + p] = ri[p];
+    } else {
+      a.out_ids[row + p] = -1;
+      a.out_dists[row + p] = 0.f;
+      if (a.out_internal) a.out_internal[row + p] = -1;
+    }
+  }
+  if (lane == 0) {
+    a.out_counts[qi] = rs;
+    if (a.stats) {
+      atomicAdd(&a.stats->expansions, nexp);
+      atomicAdd(&a.stats->dist_level0, nd0);
+      atomicAdd(&a.stats->dist_upper, ndu);
+      atomicAdd(&a.stats->list_ints_read, nl);
     }
   }
 }
@@ -640,7 +833,13 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock)
       if (lane == 0) a.out_counts[qi] = 0;
       continue;
     }
-    v1merge_query<D, FUNC, ST>(a, w, vis, ep, qi, k, ef, lane, h, ltm);
+    // Hnsw::Search dispatch (hnsw.cc:724): SearchOld for algoType=old, or
+    // hybrid with ef >= 1000
+    const bool use_old = a.algo == kAlgoOld || (a.algo == kAlgoHybrid && ef >= 1000);
+    if (use_old)
+      old_query<D, FUNC, ST>(a, w, vis, ep, qi, k, ef, lane, h, ltm, slot);
+    else
+      v1merge_query<D, FUNC, ST>(a, w, vis, ep, qi, k, ef, lane, h, ltm);
     WSYNC();
     if (a.q_ticks && lane == 0)
       a.q_ticks[qi] = (uint32_t)(__builtin_amdgcn_s_memrealtime() - t_start);

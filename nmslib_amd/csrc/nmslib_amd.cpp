@@ -133,6 +133,9 @@ struct nmsamd_index {
   DevBuf<int32_t> d_ids, d_counts, d_k, d_ef, d_internal, d_work;
   DevBuf<SearchStats> d_stats;
   DevBuf<uint32_t> d_qticks;
+  DevBuf<float> d_oldk;
+  DevBuf<int32_t> d_oldi;
+  bool pending = false;
   int64_t last_nq = 0;
   SearchStats last_stats{};
   double last_ms = 0;
@@ -346,19 +349,25 @@ int lds_wave_bytes(int cap, int dstride) {
   return (b + 15) / 16 * 16;
 }
 
-// Launches the level-0 search of nq queries whose inputs/outputs are device
-// pointers; k_of / ef_of are optional device arrays.
+// Launches the search of nq queries whose inputs/outputs are device pointers;
+// k_of / ef_of are optional device arrays whose maxima are max_k / max_ef.
 void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, const int32_t* k_of,
-                const int32_t* ef_of, int max_k, int max_cap, int32_t* ids, float* dists,
+                const int32_t* ef_of, int max_k, int max_ef, int32_t* ids, float* dists,
                 int32_t* counts, hipStream_t s) {
   if (!h->loaded) fail(NMSAMD_EINVAL, "Must call createIndex or loadIndex before this method");
-  const bool use_old = h->algo == kOld || (h->algo == kHybrid && max_cap >= 1000 && !ef_of);
-  if (use_old)
-    fail(NMSAMD_ERUNTIME, "SearchOld (ef >= 1000 or algoType=old) is not available on the device");
-  const int cap = std::max(max_cap, 1);
+  // Hnsw::Search (hnsw.cc:724): SearchOld for algoType=old or hybrid with ef >= 1000
+  const bool any_old = h->algo == kOld || (h->algo == kHybrid && max_ef >= 1000);
+  int cap = std::max(1, std::max(max_ef, max_k));
+  if (any_old) cap = std::max(cap, max_ef + 1 + max_k);
+  const bool rerank = h->storage != kF32 && h->rerank_mult > 0;
+  if (rerank) {  // staging room for the re-sorted candidates
+    const int r = std::min(std::max(max_ef, max_k), std::max(max_k, h->rerank_mult * max_k));
+    cap = std::max(cap, std::max(max_ef, max_k) + r);
+  }
   const int lds = lds_wave_bytes(cap, h->dstride);
   if ((size_t)lds > kLdsPerCu)
-    fail(NMSAMD_EINVAL, "max(ef, k) = " + std::to_string(cap) + " exceeds the LDS budget of one wave");
+    fail(NMSAMD_EINVAL, "efSearch / k too large: the search array of " + std::to_string(cap) +
+                            " items exceeds the LDS of one wave");
   ensure_lowp(h);
   // waves per block and resident blocks per CU from the LDS footprint; the
   // grid is persistent: as many resident waves as the device holds, at most
@@ -371,9 +380,16 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   const int64_t slot_cap = std::max<int64_t>(wpb, vis_budget / std::max<int64_t>(h->n, 1));
   max_blocks = std::max<int64_t>(1, std::min<int64_t>(max_blocks, slot_cap / wpb));
   const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((nq + wpb - 1) / wpb, max_blocks));
-  ensure_slots(h, blocks * wpb);
-  h->d_work.ensure(1);
-  hip_check(launch_fill_i32(h->d_work.p, 0, 1, s), "fill");
+  const int slots = blocks * wpb;
+  ensure_slots(h, slots);
+  int64_t heap_cap = 0;
+  if (any_old) {
+    heap_cap = std::max<int64_t>(65536, 64 * (int64_t)max_ef);
+    h->d_oldk.ensure((size_t)slots * heap_cap);
+    h->d_oldi.ensure((size_t)slots * heap_cap);
+  }
+  h->d_work.ensure(2);
+  hip_check(launch_fill_i32(h->d_work.p, 0, 2, s), "fill");
   if (h->collect) {
     h->d_stats.ensure(1);
     hip_check(launch_fill_i32(reinterpret_cast<int*>(h->d_stats.p), 0,
@@ -394,17 +410,30 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   a.out_counts = counts;
   a.out_internal = nullptr;
   a.work = h->d_work.p;
+  a.err = h->d_work.p + 1;
   a.stats = h->collect ? h->d_stats.p : nullptr;
   if (h->collect) h->d_qticks.ensure((size_t)nq);
   a.q_ticks = h->collect ? h->d_qticks.p : nullptr;
   h->last_nq = nq;
   a.storage = h->storage;
+  a.rerank = rerank ? h->rerank_mult : 0;
+  a.algo = h->algo == kOld ? kAlgoOld : (h->algo == kV1Merge ? kAlgoV1Merge : kAlgoHybrid);
+  a.old_heap_key = h->d_oldk.p;
+  a.old_heap_id = h->d_oldi.p;
+  a.old_heap_cap = heap_cap;
   hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
   hip_check(launch_search(a, blocks, wpb, lds, s), "search kernel launch");
   hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
+  h->pending = true;
 }
 
 void finish_stats(nmsamd_index* h) {
+  if (!h->pending) return;
+  h->pending = false;
+  int err = 0;
+  hip_check(hipMemcpy(&err, h->d_work.p + 1, sizeof(int), hipMemcpyDeviceToHost), "status");
+  if (err)
+    fail(NMSAMD_ERUNTIME, "SearchOld candidate heap overflow (more than 64*ef admitted items)");
   float ms = 0;
   hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
   h->last_ms = ms;
@@ -602,8 +631,7 @@ int nmsamd_knn_query_batch(nmsamd_index* h, const float* q, int64_t nq, int64_t 
     h->d_ids.ensure((size_t)nq * k);
     h->d_dists.ensure((size_t)nq * k);
     h->d_counts.ensure((size_t)nq);
-    const int cap = (int)std::max<size_t>(h->ef, (size_t)k);
-    run_search(h, h->d_q.p, nq, k, (int)h->ef, nullptr, nullptr, k, cap, h->d_ids.p,
+    run_search(h, h->d_q.p, nq, k, (int)h->ef, nullptr, nullptr, k, (int)h->ef, h->d_ids.p,
                h->d_dists.p, h->d_counts.p, s);
     hip_check(hipMemcpyAsync(ids, h->d_ids.p, sizeof(int32_t) * nq * k, hipMemcpyDeviceToHost, s), "D2H");
     hip_check(hipMemcpyAsync(dists, h->d_dists.p, sizeof(float) * nq * k, hipMemcpyDeviceToHost, s), "D2H");
@@ -622,8 +650,7 @@ int nmsamd_knn_query_batch_device(nmsamd_index* h, const float* d_q, int64_t nq,
     if (nq < 0 || k <= 0) fail(NMSAMD_EINVAL, "nq must be non-negative and k positive");
     if (nq == 0) return;
     hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : h->stream;
-    const int cap = (int)std::max<size_t>(h->ef, (size_t)k);
-    run_search(h, d_q, nq, k, (int)h->ef, nullptr, nullptr, k, cap, d_ids, d_dists, d_counts, s);
+    run_search(h, d_q, nq, k, (int)h->ef, nullptr, nullptr, k, (int)h->ef, d_ids, d_dists, d_counts, s);
   });
 }
 
@@ -636,10 +663,10 @@ int nmsamd_knn_query_batch_uneven(nmsamd_index* h, const float* q, int64_t nq, i
     if (!h->loaded) fail(NMSAMD_EINVAL, "Must call createIndex or loadIndex before this method");
     if (dim != h->dim) fail(NMSAMD_EINVAL, "query dimension does not match the index");
     if (nq <= 0) return;
-    int cap = 1, mk = 0;
+    int mef = 0, mk = 0;
     for (int64_t i = 0; i < nq; ++i) {
       if (k[i] < 0 || ef[i] < 0) fail(NMSAMD_EINVAL, "k and ef must be non-negative");
-      cap = std::max(cap, std::max(k[i], ef[i]));
+      mef = std::max(mef, ef[i]);
       mk = std::max(mk, k[i]);
     }
     if (mk > max_k) fail(NMSAMD_EINVAL, "max_k is smaller than a query's k");
@@ -654,7 +681,7 @@ int nmsamd_knn_query_batch_uneven(nmsamd_index* h, const float* q, int64_t nq, i
     h->d_ids.ensure((size_t)nq * max_k);
     h->d_dists.ensure((size_t)nq * max_k);
     h->d_counts.ensure((size_t)nq);
-    run_search(h, h->d_q.p, nq, 0, 0, h->d_k.p, h->d_ef.p, max_k, cap, h->d_ids.p, h->d_dists.p,
+    run_search(h, h->d_q.p, nq, 0, 0, h->d_k.p, h->d_ef.p, max_k, mef, h->d_ids.p, h->d_dists.p,
                h->d_counts.p, s);
     hip_check(hipMemcpyAsync(ids, h->d_ids.p, sizeof(int32_t) * nq * max_k, hipMemcpyDeviceToHost, s), "D2H");
     hip_check(hipMemcpyAsync(dists, h->d_dists.p, sizeof(float) * nq * max_k, hipMemcpyDeviceToHost, s), "D2H");

@@ -12,7 +12,6 @@ from tests.gpu_util import load
 
 pytestmark = pytest.mark.gpu
 
-V1_RUNS = {name: [r for r in F.runs(name) if r[2] != "old" and r[0] < 1000] for name in F.NAMES}
 
 
 def assert_same(ids, dists, counts, rids, rdists, rcounts, msg):
@@ -29,7 +28,7 @@ def test_gpu_matches_reference_goldens(name):
     idx = load(name, F.index_path(name), m["space"])
     res = F.results(name)
     q = res["queries"]
-    for ef, k, algo in V1_RUNS[name]:
+    for ef, k, algo in F.runs(name):  # V1Merge, SearchOld (old / ef >= 1000)
         idx.setQueryTimeParams({"efSearch": ef, "algoType": algo})
         ids, dists, counts = idx.knnQueryBatchArrays(q, k)
         t = F.tag(ef, k, algo)
@@ -89,12 +88,21 @@ def test_gpu_edge_cases():
     name = "tiny_l2"
     idx = load(name, F.index_path(name), "l2", 10)
     q = F.results(name)["queries"]
-    # k larger than the index: every element comes back once
+    # k above ef and above the index size: the array grows to max(ef, k) but
+    # only min(size, ef) items are expanded (hnsw_distfunc_opt.cc:200-211)
+    ids, dists, counts = idx.knnQueryBatchArrays(q[:3], 100)
+    oi, od, oc = O.OracleIndex(F.index_path(name)).search(q[:3], 100, 10)
+    np.testing.assert_array_equal(counts, oc)
+    np.testing.assert_array_equal(ids, oi)
+    for row, c in zip(ids, counts):
+        assert np.all(row[c:] == -1) and len(set(row[:c].tolist())) == c
+    # with ef >= n every element comes back once
+    idx.setQueryTimeParams({"efSearch": 100})
     ids, dists, counts = idx.knnQueryBatchArrays(q[:3], 100)
     assert np.all(counts == 60)
-    assert np.all(ids[:, 60:] == -1)
     for row in ids:
         assert sorted(row[:60].tolist()) == list(range(60))
+    idx.setQueryTimeParams({"efSearch": 10})
     # k = 0 and an empty batch
     ids, dists, counts = idx.knnQueryBatchArrays(q[:3], 0)
     assert counts.tolist() == [0, 0, 0]
