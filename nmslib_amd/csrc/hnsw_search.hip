@@ -211,27 +211,8 @@ __device__ __forceinline__ float dist_node(const float* __restrict__ qs, const D
   }
 }
 
-// number of keys[0..n) < v (lower) or <= v (upper), by binary search
-__device__ __forceinline__ int lower_count(const float* keys, int n, float v) {
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (keys[mid] < v) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
-__device__ __forceinline__ int upper_count(const float* keys, int n, float v) {
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (keys[mid] <= v) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
-
 struct WaveLds {
-  float* keys;     // [cap] sorted array keys
-  uint32_t* ids;   // [cap] internal ids | kUsedBit
+  uint2* A;        // [cap] sorted array: x = key bits, y = internal id | kUsedBit
   float* qs;       // [dstride] query
   int* cand;       // [64] fresh neighbours of this expansion (list order)
   float* cdist;    // [64] their distances
@@ -243,9 +224,8 @@ struct WaveLds {
 
 __device__ __forceinline__ WaveLds carve(unsigned char* base, int cap, int dstride) {
   WaveLds w;
-  w.keys = reinterpret_cast<float*>(base);
-  w.ids = reinterpret_cast<uint32_t*>(w.keys + cap);
-  w.qs = reinterpret_cast<float*>(w.ids + cap);
+  w.A = reinterpret_cast<uint2*>(base);
+  w.qs = reinterpret_cast<float*>(w.A + cap);
   w.cand = reinterpret_cast<int*>(w.qs + dstride);
   w.cdist = reinterpret_cast<float*>(w.cand + 64);
   w.bkey = w.cdist + 64;
@@ -253,6 +233,55 @@ __device__ __forceinline__ WaveLds carve(unsigned char* base, int cap, int dstri
   w.skey = reinterpret_cast<float*>(w.bid + 64);
   w.sid = reinterpret_cast<int*>(w.skey + 64);
   return w;
+}
+
+__device__ __forceinline__ float akey(const uint2* A, int i) { return __uint_as_float(A[i].x); }
+
+// number of keys of A[0..n) below v (lower) or not above v (upper)
+__device__ __forceinline__ int lower_count(const uint2* A, int n, float v) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (akey(A, mid) < v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+__device__ __forceinline__ int upper_count(const uint2* A, int n, float v) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (akey(A, mid) <= v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Interleaves cnt new items into A[0..na): item j (lane j) lands at pos_j + j
+// where pos_j, nondecreasing in j, is the number of A items before it; A item
+// i moves up by #{j : pos_j <= i}.  Anything landing at or beyond C drops off.
+// Chunks of 64 items move top-down: a chunk's writes land at or above its own
+// base while the next chunk reads below it, so only compiler ordering is
+// needed between them (one wave, LDS operations execute in order).
+__device__ __forceinline__ void place_items(uint2* A, int na, int C, int lane, int cnt, int pos_j,
+                                            uint2 item_j) {
+  if (cnt <= 0) return;
+  const int lo = __shfl(pos_j, 0);
+  for (int base = ((na - 1) >> 6) << 6; base >= (lo & ~63); base -= 64) {
+    const int i = base + lane;
+    const bool mv = i >= lo && i < na;
+    uint2 v = make_uint2(0u, 0u);
+    if (mv) v = A[i];
+    int sh = __popcll(__ballot(lane < cnt && pos_j < base));
+    uint64_t inm = __ballot(lane < cnt && pos_j >= base && pos_j < base + 64);
+    while (inm) {
+      const int j = __ffsll((long long)inm) - 1;
+      inm &= inm - 1;
+      sh += i >= __builtin_amdgcn_readlane(pos_j, j) ? 1 : 0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (mv && i + sh < C) A[i + sh] = v;
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (lane < cnt && pos_j + lane < C) A[pos_j + lane] = item_j;
 }
 
 // Greedy descent over levels maxlevel..1 (hnsw_distfunc_opt.cc:168-198).  A
@@ -303,6 +332,18 @@ __device__ __forceinline__ void greedy_descent(const DevIndex& ix, const float* 
   }
 }
 
+// position of the first item at or after p that is not expanded yet (n if none)
+__device__ __forceinline__ int first_unused(const uint2* A, int p, int n, int lane) {
+  while (p < n) {
+    const int i = p + lane;
+    const bool unused = i < n && !(A[i].y & kUsedBit);
+    const uint64_t um = __ballot(unused);
+    if (um) return p + __ffsll((long long)um) - 1;
+    p = min(p + 64, n);
+  }
+  return n;
+}
+
 // ---------------------------------------------------------------------------
 // SearchV1Merge at level 0 for one query held by one wave.
 template <int D, int FUNC, int ST>
@@ -310,6 +351,7 @@ __device__ void v1merge_query(const SearchArgs& a, const WaveLds& w, uint8_t* vi
                               int qi, int k, int ef, int lane, int h, uint64_t ltm) {
   const DevIndex& ix = a.ix;
   const int C = max(ef, k);
+  uint2* A = w.A;
   unsigned long long nexp = 0, nd0 = 0, ndu = 1, nl = 0, nties = 0;
 
   int cur = ix.enter;
@@ -318,25 +360,30 @@ __device__ void v1merge_query(const SearchArgs& a, const WaveLds& w, uint8_t* vi
 
   int n = 1;
   if (lane == 0) {
-    w.keys[0] = curd;
-    w.ids[0] = (uint32_t)cur;
+    A[0] = make_uint2(__float_as_uint(curd), (uint32_t)cur);
     vis[cur] = ep;
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   int ce = 0;
+  int pf_node = -1, pf_t = -1;  // adjacency row of the node predicted to expand next
   while (ce < min(n, ef)) {
     WSYNC();
-    const uint32_t e = w.ids[ce];
-    const int node = (int)(e & ~kUsedBit);
-    if (lane == 0) w.ids[ce] = e | kUsedBit;
+    const uint2 e = A[ce];
+    const int node = (int)(e.y & ~kUsedBit);
+    if (lane == 0) A[ce].y = e.y | kUsedBit;
     ++ce;
     ++nexp;
-    const float topk = w.keys[n - 1];
+    const float topk = akey(A, n - 1);
     const int nsnap = n;
 
-    // adjacency list (one 128-byte row for maxM0 <= 32)
-    const int32_t* lst = ix.l0 + (int64_t)node * ix.l0stride;
-    const int t = lane < ix.maxM0 ? lst[lane] : -1;
+    // adjacency list (one 128-byte row for maxM0 <= 32), usually prefetched
+    int t;
+    if (node == pf_node) {
+      t = pf_t;
+    } else {
+      const int32_t* lst = ix.l0 + (int64_t)node * ix.l0stride;
+      t = lane < ix.maxM0 ? lst[lane] : -1;
+    }
     const bool valid = t >= 0;
     nl += 1 + __popcll(__ballot(valid));
     bool fresh = false;
@@ -363,14 +410,43 @@ __device__ void v1merge_query(const SearchArgs& a, const WaveLds& w, uint8_t* vi
     }
     WSYNC();
     const float dj = lane < ncand ? w.cdist[lane] : 0.f;
+    const int cj = lane < ncand ? w.cand[lane] : 0;
     const bool keep = lane < ncand && (dj < topk || nsnap < ef);
     const uint64_t km = __ballot(keep);
     const int m = __popcll(km);
+
+    // The node expanded next is the first unused item at or after ce, unless
+    // a new item sorts before it: start loading its adjacency row now so the
+    // load overlaps the sort and merge below.
+    {
+      const int p = first_unused(A, ce, n, lane);
+      float nk = p < n ? akey(A, p) : kInf();
+      int nid = p < n ? (int)(A[p].y & ~kUsedBit) : -1;
+      float bk = keep ? dj : kInf();
+      int bi = keep ? cj : -1;
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) {
+        const float ok = __shfl_xor(bk, o);
+        const int oi = __shfl_xor(bi, o);
+        if (ok < bk) {
+          bk = ok;
+          bi = oi;
+        }
+      }
+      const int guess = (bi >= 0 && bk < nk) ? bi : nid;
+      if (guess >= 0 && (p < min(n, ef) || bk < nk)) {
+        pf_node = guess;
+        pf_t = lane < ix.maxM0 ? ix.l0[(int64_t)guess * ix.l0stride + lane] : -1;
+      } else {
+        pf_node = -1;
+      }
+    }
+
     if (m > 0) {
       if (keep) {
         const int r = __popcll(km & ltm);
         w.bkey[r] = dj;
-        w.bid[r] = w.cand[lane];
+        w.bid[r] = cj;
       }
       WSYNC();
       // stable ranks; an equal pair among more than 16 items needs the
@@ -401,31 +477,14 @@ __device__ void v1merge_query(const SearchArgs& a, const WaveLds& w, uint8_t* vi
       WSYNC();
       const float bj = lane < m ? w.skey[lane] : 0.f;
       const int bidj = lane < m ? w.sid[lane] : 0;
-      const int Lj = lane < m ? lower_count(w.keys, n, bj) : 0;
-      const int Uj = lane < m ? upper_count(w.keys, n, bj) : 0;
+      const int Lj = lane < m ? lower_count(A, n, bj) : 0;
+      const int Uj = lane < m ? upper_count(A, n, bj) : 0;
       const bool slow = anytie || (__ballot(lane < m && Lj != Uj) != 0);
       if (!slow) {
         // distinct keys: sequential insertion == merge truncated to C
         const int L0 = __shfl(Lj, 0);
-        const int nn = min(C, n + m);
-        for (int top = ((n - 1) >> 6) << 6; top >= ((L0 >> 6) << 6); top -= 64) {
-          const int i = top + lane;
-          const bool mv = i >= L0 && i < n;
-          const float ki = mv ? w.keys[i] : 0.f;
-          const uint32_t ii = mv ? w.ids[i] : 0u;
-          const int np = mv ? i + lower_count(w.skey, m, ki) : 0;
-          WSYNC();
-          if (mv && np < C) {
-            w.keys[np] = ki;
-            w.ids[np] = ii;
-          }
-          WSYNC();
-        }
-        if (lane < m && lane + Lj < C) {
-          w.keys[lane + Lj] = bj;
-          w.ids[lane + Lj] = (uint32_t)bidj;
-        }
-        n = nn;
+        place_items(A, n, C, lane, m, Lj, make_uint2(__float_as_uint(bj), (uint32_t)bidj));
+        n = min(C, n + m);
         if (L0 < ce) ce = L0;
       } else {
         // Equal keys: replay the reference's item-by-item insertion.  The
@@ -446,7 +505,7 @@ __device__ void v1merge_query(const SearchArgs& a, const WaveLds& w, uint8_t* vi
           const int nv = na + ns;
           const int ab_last = __shfl(s_ab, ns > 0 ? ns - 1 : 0);
           const bool last_is_s = ns > 0 && ab_last == na;
-          const float lastkey = last_is_s ? __shfl(s_key, ns - 1) : w.keys[na - 1];
+          const float lastkey = last_is_s ? __shfl(s_key, ns - 1) : akey(A, na - 1);
           int ret;
           if (lastkey <= b) {
             // append after an equal last key while there is room
@@ -495,53 +554,13 @@ __device__ void v1merge_query(const SearchArgs& a, const WaveLds& w, uint8_t* vi
           if (ret < ce) ce = ret;
         }
         // materialise: A item i moves up by the number of S items before it
-        if (lane < ns) {
-          w.bkey[lane] = s_key;
-          w.bid[lane] = s_id;
-          w.cand[lane] = s_ab;
-        }
-        WSYNC();
-        const int ab0 = ns > 0 ? __shfl(s_ab, 0) : na;
-        for (int top = ((na - 1) >> 6) << 6; top >= ((ab0 >> 6) << 6); top -= 64) {
-          const int i = top + lane;
-          const bool mv = i >= ab0 && i < na;
-          const float ki = mv ? w.keys[i] : 0.f;
-          const uint32_t ii = mv ? w.ids[i] : 0u;
-          int before = 0;
-          if (mv) {
-            int lo = 0, hi = ns;
-            while (lo < hi) {
-              const int mid = (lo + hi) >> 1;
-              if (w.cand[mid] <= i) lo = mid + 1; else hi = mid;
-            }
-            before = lo;
-          }
-          WSYNC();
-          if (mv) {
-            w.keys[i + before] = ki;
-            w.ids[i + before] = ii;
-          }
-          WSYNC();
-        }
-        if (lane < ns) {
-          w.keys[s_ab + lane] = s_key;
-          w.ids[s_ab + lane] = (uint32_t)s_id;
-        }
+        place_items(A, na, C, lane, ns, s_ab, make_uint2(__float_as_uint(s_key), (uint32_t)s_id));
         n = na + ns;
       }
     }
     WSYNC();
     // the next unexpanded item (hnsw_distfunc_opt.cc:272-273)
-    while (ce < n) {
-      const int i = ce + lane;
-      const bool unused = i < n && !(w.ids[i] & kUsedBit);
-      const uint64_t um = __ballot(unused);
-      if (um) {
-        ce += __ffsll((long long)um) - 1;
-        break;
-      }
-      ce = min(ce + 64, n);
-    }
+    ce = first_unused(A, ce, n, lane);
     // the marks of this expansion must land before the next one reads them
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   }
@@ -556,39 +575,28 @@ __device__ void v1merge_query(const SearchArgs& a, const WaveLds& w, uint8_t* vi
       for (int b = 0; b < R; b += 32) {
         const int s = b + (lane >> 1);
         const bool ok = s < R;
-        const int id = ok ? (int)(w.ids[s] & ~kUsedBit) : 0;
+        const int id = ok ? (int)(A[s].y & ~kUsedBit) : 0;
         const float dd = dist_pair_f32<D, FUNC>(w.qs, ix.vec + (int64_t)id * ix.dstride, h, ix.dim);
-        if (ok && h == 0) {
-          w.keys[s] = dd;
-          w.ids[s] = (uint32_t)id;
-        }
+        if (ok && h == 0) A[s] = make_uint2(__float_as_uint(dd), (uint32_t)id);
       }
       WSYNC();
-      // rank sort in place through registers, 64 items per round
+      // rank sort, staged in A[R, 2R) (the capacity holds R more items)
       for (int b = 0; b < R; b += 64) {
         const int p = b + lane;
-        float key = 0.f;
-        uint32_t id = 0;
-        int rank = 0;
         if (p < R) {
-          key = w.keys[p];
-          id = w.ids[p];
+          const uint2 v = A[p];
+          const float key = __uint_as_float(v.x);
+          int rank = 0;
           for (int j = 0; j < R; ++j) {
-            const float kj = w.keys[j];
-            rank += (kj < key) || (kj == key && w.ids[j] < id);
+            const uint2 o = A[j];
+            const float kj = __uint_as_float(o.x);
+            rank += (kj < key) || (kj == key && o.y < v.y);
           }
-        }
-        if (p < R) {
-          // keys[R, 2R) is free (the capacity holds R more items): stage there
-          w.keys[R + rank] = key;
-          w.ids[R + rank] = id;
+          A[R + rank] = v;
         }
         WSYNC();
       }
-      for (int p = lane; p < R; p += 64) {
-        w.keys[p] = w.keys[R + p];
-        w.ids[p] = w.ids[R + p];
-      }
+      for (int p = lane; p < R; p += 64) A[p] = A[R + p];
       WSYNC();
       n = R;
     }
@@ -600,12 +608,13 @@ __device__ void v1merge_query(const SearchArgs& a, const WaveLds& w, uint8_t* vi
   for (int b = 0; b < k; b += 64) {
     const int p = b + lane;
     if (p < kk) {
-      const float key = w.keys[p];
-      const uint32_t id = w.ids[p] & ~kUsedBit;
-      const int start = lower_count(w.keys, kk, key);
+      const uint2 v = A[p];
+      const float key = __uint_as_float(v.x);
+      const uint32_t id = v.y & ~kUsedBit;
+      const int start = lower_count(A, kk, key);
       int rank = start;
-      for (int j = start; j < kk && w.keys[j] == key; ++j)
-        rank += ((w.ids[j] & ~kUsedBit) < id) ? 1 : 0;
+      for (int j = start; j < kk && akey(A, j) == key; ++j)
+        rank += ((A[j].y & ~kUsedBit) < id) ? 1 : 0;
       a.out_ids[row + rank] = ix.ext_id[id];
       a.out_dists[row + rank] = key;
       if (a.out_internal) a.out_internal[row + rank] = (int32_t)id;
@@ -643,10 +652,11 @@ __device__ void old_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, u
   unsigned long long nexp = 0, nd0 = 0, ndu = 1, nl = 0;
   float* hk = a.old_heap_key + slot * a.old_heap_cap;   // candidate heap keys (-d)
   int32_t* hi = a.old_heap_id + slot * a.old_heap_cap;
-  float* bk = w.keys;                                   // ef-heap (max on d)
-  int32_t* bi = reinterpret_cast<int32_t*>(w.ids);
-  float* rk = w.keys + (ef + 1);                        // result queue, ascending
-  int32_t* ri = reinterpret_cast<int32_t*>(w.ids) + (ef + 1);
+  // the wave's sorted-array storage (cap >= ef + 1 + k items of 8 bytes)
+  float* bk = reinterpret_cast<float*>(w.A);            // ef-heap (max on d)
+  int32_t* bi = reinterpret_cast<int32_t*>(bk + (ef + 1));
+  float* rk = reinterpret_cast<float*>(bi + (ef + 1));  // result queue, ascending
+  int32_t* ri = reinterpret_cast<int32_t*>(rk + k);
 
   int cur = ix.enter;
   float curd = dist_node<D, FUNC, ST>(w.qs, ix, cur, h);
