@@ -30,7 +30,8 @@ sys.path.insert(0, ROOT)
 WORKLOAD = "halfnormal1m"
 PACK = os.path.join(ROOT, "workloads", "halfnormal1m_M16_efc500.pack.npz")
 GT = os.path.join(ROOT, "workloads", "halfnormal1m_gt.npz")
-EF_LADDER = [4608, 4736, 4864, 4992, 5120, 5632, 6144, 7168, 8192, 10240, 12288]
+EF_LADDER = [2560, 2816, 3072, 3328, 3584, 3840, 4096, 4352, 4608, 4736, 4864, 4992, 5120,
+             5632, 6144, 7168, 8192, 10240, 12288]
 SWEEP = [16, 32, 64, 128, 256, 512, 1024, 2048, 4096]
 HBM_PEAK_GBS = 8000.0
 
@@ -122,6 +123,8 @@ def main():
                     "reaching recall@10 >= 0.95)")
     ap.add_argument("--algo", default="v1merge")
     ap.add_argument("--storage", default="fp32", choices=["fp32", "bf16", "fp8"])
+    ap.add_argument("--rerank", type=int, default=0, help="FP32 rerank width as a multiple of k")
+    ap.add_argument("--beam", type=int, default=0, help="0: exact V1Merge; 1..4: beam mode")
     ap.add_argument("--nq", type=int, default=10000)
     ap.add_argument("--sweep", action="store_true", help="also print the recall/QPS curve")
     ap.add_argument("--no-cpu", action="store_true")
@@ -148,7 +151,8 @@ def main():
     idx.loadIndex(path)
     log(f"[bench] loadIndex {time.time() - t:.1f}s {idx.info()}")
     if args.storage != "fp32":
-        idx.setPrecision(args.storage, 0)
+        idx.setPrecision(args.storage, args.rerank)
+    idx.setBeam(args.beam)
     dev = torch.device("cuda:0")
     dq = torch.from_numpy(q).to(dev)
     dids = torch.empty((args.nq, k), dtype=torch.int32, device=dev)
@@ -242,6 +246,7 @@ def main():
         "config": {"workload": "1,000,000 x 128 l2 half-normal u8-valued, M=16 efC=500, "
                                "10,000 queries, k=10",
                    "efSearch": ef, "algoType": args.algo, "storage": args.storage,
+                   "rerank": args.rerank, "mode": "exact V1Merge" if not args.beam else f"beam {args.beam}",
                    "recall_at_10": round(recall, 4),
                    "latency_us_p50": round(float(np.percentile(lat, 50)), 1),
                    "latency_us_p99": round(float(np.percentile(lat, 99)), 1),

@@ -94,6 +94,12 @@ int nmsamd_index_info(const nmsamd_index* h, int64_t* out8);
  * exactly in FP32.  No reference counterpart (extension of the query path). */
 int nmsamd_set_precision(nmsamd_index* h, int storage, int rerank_mult);
 
+/* Search mode: 0 = the reference's SearchV1Merge replayed exactly (default);
+ * 1..4 = beam mode, that many unexpanded items expanded per step with ties
+ * ordered by internal id (no reference counterpart; same array, capacity and
+ * stop rule, fewer serial steps).  SearchOld is unaffected. */
+int nmsamd_set_beam(nmsamd_index* h, int beam);
+
 /* Per-call counters of the last search (expansions, level-0 distance
  * evaluations, upper-level distance evaluations, adjacency ints read,
  * expansions that took the tie path) and the kernel time of the last search
@@ -108,6 +114,34 @@ int nmsamd_collect_stats(nmsamd_index* h, int on);
  * microseconds (first nq queries).  No reference counterpart (the
  * reference times queries on the host, experiments.h:186-194). */
 int nmsamd_last_query_us(const nmsamd_index* h, float* out_us, int64_t nq);
+
+/* ---- continuous-batching server ------------------------------------------
+ * No reference counterpart in the library; it stands where the reference's
+ * Thrift query server (query_server/cpp_client_server/QueryService_server.cpp:
+ * 411-466) answers knnQueryBatch requests.  A server owns `lanes` independent
+ * launch slots over a loaded index (own stream, pinned staging, device
+ * buffers and visited-mark slots), so batches submitted to different lanes
+ * overlap on the device.  Each batch carries per-query k and ef (1 <= k <=
+ * max_k, 0 <= ef <= max_ef) and runs V1Merge (or the index's beam mode /
+ * precision settings at open time).  use_graph captures each lane's launch
+ * as a hipGraph once and replays it per batch.  The index handle must not be
+ * reconfigured while a server is open on it. */
+typedef struct nmsamd_server nmsamd_server;
+
+int nmsamd_server_open(nmsamd_index* h, int lanes, int max_queries, int max_ef, int max_k,
+                       int use_graph, nmsamd_server** out);
+/* Copies the batch (q [nq][dim], k[nq], ef[nq]) and launches it on `lane`;
+ * returns at once.  The lane must be idle. */
+int nmsamd_server_submit(nmsamd_server* s, int lane, const float* q, int64_t nq, const int32_t* k,
+                         const int32_t* ef);
+/* 1 when the lane's batch is complete (or the lane is idle), 0 while running. */
+int nmsamd_server_poll(nmsamd_server* s, int lane);
+/* Copies the completed batch's results out ([nq][max_k] rows, counts[nq])
+ * and frees the lane. */
+int nmsamd_server_fetch(nmsamd_server* s, int lane, int32_t* ids, float* dists, int32_t* counts);
+/* lanes, max_queries, max_ef, max_k, slots per lane, waves per block, graph, wide */
+int nmsamd_server_info(const nmsamd_server* s, int64_t* out8);
+int nmsamd_server_close(nmsamd_server* s);
 
 #ifdef __cplusplus
 }

@@ -65,6 +65,7 @@ SIGNATURES = {
     "nmsamd_last_stats": (C.c_int, [_P, _P, C.POINTER(C.c_double)]),
     "nmsamd_collect_stats": (C.c_int, [_P, C.c_int]),
     "nmsamd_last_query_us": (C.c_int, [_P, _P, _I64]),
+    "nmsamd_set_beam": (C.c_int, [_P, C.c_int]),
 }
 
 
@@ -175,6 +176,10 @@ class FloatIndex:
         FP32 rerank width as a multiple of k (0 = no rerank)."""
         code = {"fp32": 0, "bf16": 1, "fp8": 2}[storage]
         _check(load_library().nmsamd_set_precision(self._h, code, int(rerank_mult)))
+
+    def setBeam(self, beam: int = 0):
+        """0: exact V1Merge (the reference's results); 1..4: beam mode."""
+        _check(load_library().nmsamd_set_beam(self._h, int(beam)))
 
     def collectStats(self, on: bool = True):
         _check(load_library().nmsamd_collect_stats(self._h, int(bool(on))))

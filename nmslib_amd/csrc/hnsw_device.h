@@ -80,6 +80,10 @@ struct SearchArgs {
   SearchStats* stats;        // optional
   int storage;               // Storage
   int rerank;                // > 0: exact FP32 rerank of the best rerank*k candidates
+  int beam;                  // > 0: beam mode, expansions per step (extension)
+  int64_t slot_base;         // first visited-mark slot of this launch
+  const int64_t* nq_dev;     // optional: the query count read on the device
+                             // (a replayed graph keeps its kernel arguments)
   int algo;                  // AlgoMode
   float* old_heap_key;       // SearchOld candidate heap, old_heap_cap per slot
   int32_t* old_heap_id;

@@ -1,4 +1,6 @@
 // HNSW k-NN search kernels for CDNA4 (gfx950): one wavefront per query.
+// (Header of device code; search_f32.hip / search_bf16.hip / search_fp8.hip
+// instantiate it per vector storage type.)
 //
 // Semantics follow the reference's optimized search
 // (similarity_search/src/method/hnsw_distfunc_opt.cc):
@@ -30,6 +32,7 @@
 //
 // This file is compiled with -ffp-contract=off: FP32 distances must not be
 // contracted into FMAs (the reference rounds the product before the add).
+#pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -211,27 +214,248 @@ __device__ __forceinline__ float dist_node(const float* __restrict__ qs, const D
   }
 }
 
+// Batched distances of cand[0..ncand) with every load of a round issued before
+// any arithmetic: each lane pair owns NV vectors per round, so a beam step's
+// ~100 gathers cost one or two memory round trips instead of one per 32
+// vectors.  Registers per lane and vector: 4*(D/16)*... = D/2 floats (FP32),
+// D/4 (BF16), D/8 (FP8) -- NV is chosen so the round stays near 128 VGPRs.
+// The FP32 arithmetic is dist_pair_f32's (the reference's AVX order).
+template <int D, int ST>
+struct VecRegs;
+template <int D>
+struct VecRegs<D, kF32> {
+  static constexpr int Q16 = D / 16, R4 = (D % 16) / 4;
+  float4 c[2 * Q16 + (R4 > 0 ? R4 : 0) + (2 * Q16 + R4 == 0 ? 1 : 0)];
+};
+template <int D>
+struct VecRegs<D, kBF16> {
+  static constexpr int Q16 = D / 16, T = D % 16;
+  uint4 c[Q16 > 0 ? Q16 : 1];
+  uint16_t t[(T + 1) / 2 > 0 ? (T + 1) / 2 : 1];
+};
+template <int D>
+struct VecRegs<D, kFP8> {
+  static constexpr int Q16 = D / 16, T = D % 16;
+  uint2 c[Q16 > 0 ? Q16 : 1];
+  uint8_t t[(T + 1) / 2 > 0 ? (T + 1) / 2 : 1];
+  float scale;
+};
+
+template <int D, int ST>
+__device__ __forceinline__ void vec_load(VecRegs<D, ST>& r, const DevIndex& ix, int64_t node, int h) {
+  constexpr int Q16 = D / 16;
+  if constexpr (ST == kF32) {
+    const float* x = ix.vec + node * ix.dstride;
+#pragma unroll
+    for (int c = 0; c < Q16; ++c) {
+      r.c[2 * c] = *reinterpret_cast<const float4*>(x + 16 * c + 4 * h);
+      r.c[2 * c + 1] = *reinterpret_cast<const float4*>(x + 16 * c + 8 + 4 * h);
+    }
+#pragma unroll
+    for (int t = 0; t < VecRegs<D, kF32>::R4; ++t)
+      r.c[2 * Q16 + t] = *reinterpret_cast<const float4*>(x + 16 * Q16 + 4 * t);
+  } else if constexpr (ST == kBF16) {
+    const uint16_t* x = ix.vec16 + node * ix.dstride;
+#pragma unroll
+    for (int c = 0; c < Q16; ++c) r.c[c] = *reinterpret_cast<const uint4*>(x + 16 * c + 8 * h);
+#pragma unroll
+    for (int t = 0; 2 * t + h < D % 16; ++t) r.t[t] = x[16 * Q16 + 2 * t + h];
+  } else {
+    const uint8_t* x = ix.vec8 + node * ix.dstride;
+#pragma unroll
+    for (int c = 0; c < Q16; ++c) r.c[c] = *reinterpret_cast<const uint2*>(x + 16 * c + 8 * h);
+#pragma unroll
+    for (int t = 0; 2 * t + h < D % 16; ++t) r.t[t] = x[16 * Q16 + 2 * t + h];
+    r.scale = ix.scale8[node];
+  }
+}
+
+template <int D, int FUNC, int ST>
+__device__ __forceinline__ float vec_dist(const VecRegs<D, ST>& r, const float* __restrict__ qs,
+                                          int h) {
+  constexpr int Q16 = D / 16;
+  if constexpr (ST == kF32) {
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+#pragma unroll
+    for (int c = 0; c < Q16; ++c) {
+      const float4 x0 = r.c[2 * c], x1 = r.c[2 * c + 1];
+      const float4 q0 = *reinterpret_cast<const float4*>(qs + 16 * c + 4 * h);
+      const float4 q1 = *reinterpret_cast<const float4*>(qs + 16 * c + 8 + 4 * h);
+      a0 = a0 + term<FUNC>(q0.x, x0.x);
+      a1 = a1 + term<FUNC>(q0.y, x0.y);
+      a2 = a2 + term<FUNC>(q0.z, x0.z);
+      a3 = a3 + term<FUNC>(q0.w, x0.w);
+      a0 = a0 + term<FUNC>(q1.x, x1.x);
+      a1 = a1 + term<FUNC>(q1.y, x1.y);
+      a2 = a2 + term<FUNC>(q1.z, x1.z);
+      a3 = a3 + term<FUNC>(q1.w, x1.w);
+    }
+    const float p0 = __shfl_xor(a0, 1), p1 = __shfl_xor(a1, 1);
+    const float p2 = __shfl_xor(a2, 1), p3 = __shfl_xor(a3, 1);
+    const float l0 = h ? p0 : a0, l1 = h ? p1 : a1, l2 = h ? p2 : a2, l3 = h ? p3 : a3;
+    const float u0 = h ? a0 : p0, u1 = h ? a1 : p1, u2 = h ? a2 : p2, u3 = h ? a3 : p3;
+    float s;
+    if constexpr (FUNC == kL2Sqr16) {
+      s = l0 + l1;
+      s = s + l2;
+      s = s + l3;
+      s = s + u0;
+      s = s + u1;
+      s = s + u2;
+      s = s + u3;
+    } else {
+      float s0 = l0 + u0, s1 = l1 + u1, s2 = l2 + u2, s3 = l3 + u3;
+#pragma unroll
+      for (int t = 0; t < VecRegs<D, kF32>::R4; ++t) {
+        const float4 x4 = r.c[2 * Q16 + t];
+        const float4 q4 = *reinterpret_cast<const float4*>(qs + 16 * Q16 + 4 * t);
+        s0 = s0 + term<FUNC>(q4.x, x4.x);
+        s1 = s1 + term<FUNC>(q4.y, x4.y);
+        s2 = s2 + term<FUNC>(q4.z, x4.z);
+        s3 = s3 + term<FUNC>(q4.w, x4.w);
+      }
+      s = s0 + s1;
+      s = s + s2;
+      s = s + s3;
+    }
+    return finish<FUNC>(s);
+  } else {
+    float acc = 0.f;
+    const float scale = [&] {
+      if constexpr (ST == kFP8) return r.scale; else return 1.f;
+    }();
+#pragma unroll
+    for (int c = 0; c < Q16; ++c) {
+      const int e0 = 16 * c + 8 * h;
+      float xv[8];
+      if constexpr (ST == kBF16) {
+        const uint32_t wv[4] = {r.c[c].x, r.c[c].y, r.c[c].z, r.c[c].w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          xv[2 * j] = bf16_to_f32(wv[j] & 0xffffu);
+          xv[2 * j + 1] = bf16_to_f32(wv[j] >> 16);
+        }
+      } else {
+        const uint32_t wv[2] = {r.c[c].x, r.c[c].y};
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const auto lo = __builtin_amdgcn_cvt_pk_f32_fp8((int)wv[j], false);
+          const auto hi = __builtin_amdgcn_cvt_pk_f32_fp8((int)wv[j], true);
+          xv[4 * j] = lo[0] * scale;
+          xv[4 * j + 1] = lo[1] * scale;
+          xv[4 * j + 2] = hi[0] * scale;
+          xv[4 * j + 3] = hi[1] * scale;
+        }
+      }
+      const float4 qa = *reinterpret_cast<const float4*>(qs + e0);
+      const float4 qb = *reinterpret_cast<const float4*>(qs + e0 + 4);
+      const float qv[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        if constexpr (FUNC == kL2Sqr16 || FUNC == kL2Sqr) {
+          const float t = qv[j] - xv[j];
+          acc = __builtin_fmaf(t, t, acc);
+        } else {
+          acc = __builtin_fmaf(qv[j], xv[j], acc);
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; 2 * t + h < D % 16; ++t) {
+      const int c = 16 * Q16 + 2 * t + h;
+      float xv;
+      if constexpr (ST == kBF16) {
+        xv = bf16_to_f32(r.t[t]);
+      } else {
+        xv = __builtin_amdgcn_cvt_pk_f32_fp8((int)r.t[t], false)[0] * scale;
+      }
+      if constexpr (FUNC == kL2Sqr16 || FUNC == kL2Sqr) {
+        const float d = qs[c] - xv;
+        acc = __builtin_fmaf(d, d, acc);
+      } else {
+        acc = __builtin_fmaf(qs[c], xv, acc);
+      }
+    }
+    acc = acc + __shfl_xor(acc, 1);
+    return finish<FUNC>(acc);
+  }
+}
+
+template <int ST>
+struct WideNV {
+  static constexpr int value = ST == kF32 ? 2 : 4;
+};
+
+// cdist[s] = distance of cand[s] for s < ncand (fallback: a valid node id used
+// by idle lane pairs so every shuffle sees all lanes)
+template <int D, int FUNC, int ST, int WIDE>
+__device__ __forceinline__ void dist_cands(const float* __restrict__ qs, const DevIndex& ix,
+                                           const int* cand, float* cdist, int ncand, int lane,
+                                           int h, int fallback) {
+  if constexpr (WIDE && D > 0 && D % 4 == 0) {
+    constexpr int NV = WideNV<ST>::value;
+    for (int b = 0; b < ncand; b += 32 * NV) {
+      VecRegs<D, ST> r[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const int s = b + 32 * v + (lane >> 1);
+        const int id = s < ncand ? cand[s] : fallback;
+        vec_load<D, ST>(r[v], ix, id, h);
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const int s = b + 32 * v + (lane >> 1);
+        const float dd = vec_dist<D, FUNC, ST>(r[v], qs, h);
+        if (s < ncand && h == 0) cdist[s] = dd;
+      }
+    }
+  } else {
+    for (int b = 0; b < ncand; b += 32) {
+      const int s = b + (lane >> 1);
+      const bool ok = s < ncand;
+      const int id = ok ? cand[s] : fallback;
+      const float dd = dist_node<D, FUNC, ST>(qs, ix, id, h);
+      if (ok && h == 0) cdist[s] = dd;
+    }
+  }
+}
+
+constexpr int kScratch = 128;  // per-step buffers: up to 128 new neighbours
+constexpr int kHashSlots = 256;
+
+// Per-wave LDS: the sorted array, the query, and two 1 KB scratch regions
+// whose uses do not overlap in time within a step:
+//   region X: dedupe table -> fresh neighbours + distances -> sorted buffer
+//   region Y: picks of the beam step -> filtered buffer (list order)
 struct WaveLds {
   uint2* A;        // [cap] sorted array: x = key bits, y = internal id | kUsedBit
   float* qs;       // [dstride] query
-  int* cand;       // [64] fresh neighbours of this expansion (list order)
-  float* cdist;    // [64] their distances
-  float* bkey;     // [64] buffer (list order)
-  int* bid;
-  float* skey;     // [64] buffer (sorted)
-  int* sid;
+  uint32_t* htab;  // X [kHashSlots] per-step dedupe table of the beam search
+  int* cand;       // X [kScratch] fresh neighbours of this step (list order)
+  float* cdist;    // X [kScratch] their distances
+  float* skey;     // X [kScratch] buffer (sorted)
+  int* sid;        // X
+  int* picks;      // Y [beam] nodes expanded by a beam step
+  float* bkey;     // Y [kScratch] buffer (list order)
+  int* bid;        // Y
 };
+
+constexpr int kScratchBytes = 2 * kScratch * 8;  // regions X and Y
 
 __device__ __forceinline__ WaveLds carve(unsigned char* base, int cap, int dstride) {
   WaveLds w;
   w.A = reinterpret_cast<uint2*>(base);
-  w.qs = reinterpret_cast<float*>(w.A + cap);
-  w.cand = reinterpret_cast<int*>(w.qs + dstride);
-  w.cdist = reinterpret_cast<float*>(w.cand + 64);
-  w.bkey = w.cdist + 64;
-  w.bid = reinterpret_cast<int*>(w.bkey + 64);
-  w.skey = reinterpret_cast<float*>(w.bid + 64);
-  w.sid = reinterpret_cast<int*>(w.skey + 64);
+  w.qs = reinterpret_cast<float*>(w.A + ((cap + 1) & ~1));  // keeps 16-byte alignment
+  int* x = reinterpret_cast<int*>(w.qs + dstride);
+  int* y = x + 2 * kScratch;
+  w.htab = reinterpret_cast<uint32_t*>(x);
+  w.cand = x;
+  w.cdist = reinterpret_cast<float*>(x + kScratch);
+  w.skey = reinterpret_cast<float*>(x);
+  w.sid = x + kScratch;
+  w.picks = y;
+  w.bkey = reinterpret_cast<float*>(y);
+  w.bid = y + kScratch;
   return w;
 }
 
@@ -265,11 +489,8 @@ __device__ __forceinline__ void place_items(uint2* A, int na, int C, int lane, i
                                             uint2 item_j) {
   if (cnt <= 0) return;
   const int lo = __shfl(pos_j, 0);
-  for (int base = ((na - 1) >> 6) << 6; base >= (lo & ~63); base -= 64) {
-    const int i = base + lane;
-    const bool mv = i >= lo && i < na;
-    uint2 v = make_uint2(0u, 0u);
-    if (mv) v = A[i];
+  const int lo_chunk = lo & ~63;
+  auto shift_of = [&](int i, int base) {
     int sh = __popcll(__ballot(lane < cnt && pos_j < base));
     uint64_t inm = __ballot(lane < cnt && pos_j >= base && pos_j < base + 64);
     while (inm) {
@@ -277,8 +498,22 @@ __device__ __forceinline__ void place_items(uint2* A, int na, int C, int lane, i
       inm &= inm - 1;
       sh += i >= __builtin_amdgcn_readlane(pos_j, j) ? 1 : 0;
     }
+    return sh;
+  };
+  // two chunks per round: both are read before either is written
+  for (int base = ((na - 1) >> 6) << 6; base >= lo_chunk; base -= 128) {
+    const int b1 = base - 64;
+    const int i0 = base + lane, i1 = b1 + lane;
+    const bool mv0 = i0 >= lo && i0 < na;
+    const bool mv1 = b1 >= lo_chunk && i1 >= lo && i1 < na;
+    uint2 v0 = make_uint2(0u, 0u), v1 = make_uint2(0u, 0u);
+    if (mv0) v0 = A[i0];
+    if (mv1) v1 = A[i1];
+    const int sh0 = shift_of(i0, base);
+    const int sh1 = b1 >= lo_chunk ? shift_of(i1, b1) : 0;
     __builtin_amdgcn_wave_barrier();
-    if (mv && i + sh < C) A[i + sh] = v;
+    if (mv0 && i0 + sh0 < C) A[i0 + sh0] = v0;
+    if (mv1 && i1 + sh1 < C) A[i1 + sh1] = v1;
     __builtin_amdgcn_wave_barrier();
   }
   if (lane < cnt && pos_j + lane < C) A[pos_j + lane] = item_j;
@@ -342,6 +577,87 @@ __device__ __forceinline__ int first_unused(const uint2* A, int p, int n, int la
     p = min(p + 64, n);
   }
   return n;
+}
+
+struct QueryCounters {
+  unsigned long long nexp, nd0, ndu, nl, nties;
+};
+
+// Optional FP32 rerank, then the first min(k, n) items of the sorted array in
+// (distance, internal id) order, and the query's counters.
+template <int D, int FUNC, int ST>
+__device__ void finish_query(const SearchArgs& a, const WaveLds& w, int n, int qi, int k,
+                             int lane, int h, const QueryCounters& c) {
+  const DevIndex& ix = a.ix;
+  uint2* A = w.A;
+  if constexpr (ST != kF32) {
+    // exact FP32 rerank of the best R candidates of the reduced-precision
+    // traversal: distances recomputed from the FP32 vectors in the reference's
+    // summation order, then the R items re-sorted by (distance, internal id)
+    if (a.rerank > 0) {
+      const int R = min(n, max(k, a.rerank * k));
+      for (int b = 0; b < R; b += 32) {
+        const int s = b + (lane >> 1);
+        const bool ok = s < R;
+        const int id = ok ? (int)(A[s].y & ~kUsedBit) : 0;
+        const float dd = dist_pair_f32<D, FUNC>(w.qs, ix.vec + (int64_t)id * ix.dstride, h, ix.dim);
+        if (ok && h == 0) A[s] = make_uint2(__float_as_uint(dd), (uint32_t)id);
+      }
+      WSYNC();
+      // rank sort, staged in A[R, 2R) (the capacity holds R more items)
+      for (int b = 0; b < R; b += 64) {
+        const int p = b + lane;
+        if (p < R) {
+          const uint2 v = A[p];
+          const float key = __uint_as_float(v.x);
+          int rank = 0;
+          for (int j = 0; j < R; ++j) {
+            const uint2 o = A[j];
+            const float kj = __uint_as_float(o.x);
+            rank += (kj < key) || (kj == key && o.y < v.y);
+          }
+          A[R + rank] = v;
+        }
+        WSYNC();
+      }
+      for (int p = lane; p < R; p += 64) A[p] = A[R + p];
+      WSYNC();
+      n = R;
+    }
+  }
+
+  // report the first min(k, n) items, ties by internal index
+  const int kk = min(k, n);
+  const int64_t row = (int64_t)qi * a.out_stride;
+  for (int b = 0; b < k; b += 64) {
+    const int p = b + lane;
+    if (p < kk) {
+      const uint2 v = A[p];
+      const float key = __uint_as_float(v.x);
+      const uint32_t id = v.y & ~kUsedBit;
+      const int start = lower_count(A, kk, key);
+      int rank = start;
+      for (int j = start; j < kk && akey(A, j) == key; ++j)
+        rank += ((A[j].y & ~kUsedBit) < id) ? 1 : 0;
+      a.out_ids[row + rank] = ix.ext_id[id];
+      a.out_dists[row + rank] = key;
+      if (a.out_internal) a.out_internal[row + rank] = (int32_t)id;
+    } else if (p < k) {
+      a.out_ids[row + p] = -1;
+      a.out_dists[row + p] = 0.f;
+      if (a.out_internal) a.out_internal[row + p] = -1;
+    }
+  }
+  if (lane == 0) {
+    a.out_counts[qi] = kk;
+    if (a.stats) {
+      atomicAdd(&a.stats->expansions, c.nexp);
+      atomicAdd(&a.stats->dist_level0, c.nd0);
+      atomicAdd(&a.stats->dist_upper, c.ndu);
+      atomicAdd(&a.stats->list_ints_read, c.nl);
+      atomicAdd(&a.stats->tie_paths, c.nties);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -455,13 +771,12 @@ __device__ void v1merge_query(const SearchArgs& a, const WaveLds& w, uint8_t* vi
       const int mid = lane < m ? w.bid[lane] : 0;
       int rank = 0;
       bool tie = false;
-      if (lane < m) {
-        for (int j = 0; j < m; ++j) {
-          const float kj = w.bkey[j];
-          rank += (kj < mk) || (kj == mk && j < lane);
-          tie |= (kj == mk) && (j != lane);
-        }
+      for (int j = 0; j < m; ++j) {  // uniform loop: keys broadcast by readlane
+        const float kj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mk), j));
+        rank += (kj < mk) || (kj == mk && j < lane);
+        tie |= (kj == mk) && (j != lane);
       }
+      tie = tie && lane < m;
       const bool anytie = __ballot(tie) != 0;
       if (anytie && m > 16) {
         if (lane < m) {
@@ -566,74 +881,184 @@ __device__ void v1merge_query(const SearchArgs& a, const WaveLds& w, uint8_t* vi
   }
   WSYNC();
 
-  if constexpr (ST != kF32) {
-    // exact FP32 rerank of the best R candidates of the reduced-precision
-    // traversal: distances recomputed from the FP32 vectors in the reference's
-    // summation order, then the R items re-sorted by (distance, internal id)
-    if (a.rerank > 0) {
-      const int R = min(n, max(k, a.rerank * k));
-      for (int b = 0; b < R; b += 32) {
-        const int s = b + (lane >> 1);
-        const bool ok = s < R;
-        const int id = ok ? (int)(A[s].y & ~kUsedBit) : 0;
-        const float dd = dist_pair_f32<D, FUNC>(w.qs, ix.vec + (int64_t)id * ix.dstride, h, ix.dim);
-        if (ok && h == 0) A[s] = make_uint2(__float_as_uint(dd), (uint32_t)id);
+  const QueryCounters c{nexp, nd0, ndu, nl, nties};
+  finish_query<D, FUNC, ST>(a, w, n, qi, k, lane, h, c);
+}
+
+
+// ---------------------------------------------------------------------------
+// Beam mode (an extension; no reference counterpart): each step expands the
+// first `beam` unexpanded items of the sorted array together, so one round of
+// adjacency, visited and vector loads serves several expansions and the
+// serial chain of a query shortens by that factor.  Items are ordered by
+// (distance, internal id), which makes every key distinct, so new items always
+// take the parallel merge.  The array, its capacity max(ef, k) and the stop
+// rule are V1Merge's.
+__device__ __forceinline__ bool kid_less(float ka, uint32_t ia, float kb, uint32_t ib) {
+  return ka < kb || (ka == kb && ia < ib);
+}
+
+__device__ __forceinline__ int lower_count_kid(const uint2* A, int n, float v, uint32_t id) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    const uint2 e = A[mid];
+    if (kid_less(__uint_as_float(e.x), e.y & ~kUsedBit, v, id)) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+template <int D, int FUNC, int ST, int WIDE>
+__device__ void beam_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, uint8_t ep,
+                           int qi, int k, int ef, int lane, int h, uint64_t ltm) {
+  const DevIndex& ix = a.ix;
+  const int C = max(ef, k);
+  const int W = max(1, min(a.beam, kScratch / ix.maxM0));
+  uint2* A = w.A;
+  unsigned long long nexp = 0, nd0 = 0, ndu = 1, nl = 0;
+
+  int cur = ix.enter;
+  float curd = dist_node<D, FUNC, ST>(w.qs, ix, cur, h);
+  greedy_descent<D, FUNC, ST>(ix, w.qs, lane, h, cur, curd, ndu, nl);
+
+  int n = 1;
+  if (lane == 0) {
+    A[0] = make_uint2(__float_as_uint(curd), (uint32_t)cur);
+    vis[cur] = ep;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  int ce = 0;
+  while (ce < min(n, ef)) {
+    WSYNC();
+    // the first W unexpanded items below min(n, ef), in position order
+    const int lim = min(n, ef);
+    int nsel = 0;
+    for (int p = ce; p < lim && nsel < W; p += 64) {
+      const int i = p + lane;
+      const bool un = i < lim && !(A[i].y & kUsedBit);
+      const uint64_t um = __ballot(un);
+      const int take = min(__popcll(um), W - nsel);
+      const int r = __popcll(um & ltm);
+      if (un && r < take) {
+        const uint32_t y = A[i].y;
+        A[i].y = y | kUsedBit;
+        w.picks[nsel + r] = (int)y;
+      }
+      nsel += take;
+    }
+    nexp += nsel;
+    const float topk = akey(A, n - 1);
+    const int nsnap = n;
+    // clear the dedupe table while the picks land
+    reinterpret_cast<uint4*>(w.htab)[lane] = make_uint4(~0u, ~0u, ~0u, ~0u);
+    WSYNC();
+
+    // adjacency rows of the picks, two entries per lane (W * maxM0 <= 128)
+    int t[2];
+    bool fresh[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int e = lane + 64 * r;
+      const int li = e / ix.maxM0;
+      const int sl = e - li * ix.maxM0;
+      t[r] = li < nsel ? ix.l0[(int64_t)w.picks[li] * ix.l0stride + sl] : -1;
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const bool valid = t[r] >= 0;
+      nl += __popcll(__ballot(valid));
+      fresh[r] = valid && vis[t[r]] != ep;
+    }
+    nl += nsel;
+    // a neighbour shared by two picks is taken once (LDS hash, linear probing)
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      if (fresh[r]) {
+        uint32_t slot = ((uint32_t)t[r] * 2654435761u) >> 24;
+        for (;;) {
+          const uint32_t old = atomicCAS(&w.htab[slot], ~0u, (uint32_t)t[r]);
+          if (old == ~0u) break;
+          if (old == (uint32_t)t[r]) {
+            fresh[r] = false;
+            break;
+          }
+          slot = (slot + 1) & (kHashSlots - 1);
+        }
+      }
+      if (fresh[r]) vis[t[r]] = ep;
+    }
+    const uint64_t f0 = __ballot(fresh[0]);
+    const uint64_t f1 = __ballot(fresh[1]);
+    const int c0 = __popcll(f0);
+    const int ncand = c0 + __popcll(f1);
+    if (fresh[0]) w.cand[__popcll(f0 & ltm)] = t[0];
+    if (fresh[1]) w.cand[c0 + __popcll(f1 & ltm)] = t[1];
+    nd0 += ncand;
+    WSYNC();
+    dist_cands<D, FUNC, ST, WIDE>(w.qs, ix, w.cand, w.cdist, ncand, lane, h, cur);
+    WSYNC();
+    // keep what the snapshot admits; order does not matter (sorted next)
+    int m = 0;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int s = lane + 64 * r;
+      const float dj = s < ncand ? w.cdist[s] : 0.f;
+      const bool keep = s < ncand && (dj < topk || nsnap < ef);
+      const uint64_t km = __ballot(keep);
+      if (keep) {
+        const int o = m + __popcll(km & ltm);
+        w.bkey[o] = dj;
+        w.bid[o] = w.cand[s];
+      }
+      m += __popcll(km);
+    }
+    WSYNC();
+    if (m > 0) {
+      // rank sort by (distance, id); keys broadcast by readlane
+      const float k0 = lane < m ? w.bkey[lane] : 0.f;
+      const uint32_t i0 = lane < m ? (uint32_t)w.bid[lane] : 0u;
+      const float k1 = lane + 64 < m ? w.bkey[lane + 64] : 0.f;
+      const uint32_t i1 = lane + 64 < m ? (uint32_t)w.bid[lane + 64] : 0u;
+      int r0 = 0, r1 = 0;
+      for (int j = 0; j < m; ++j) {
+        const bool lo = j < 64;
+        const float kj = __int_as_float(
+            __builtin_amdgcn_readlane(__float_as_int(lo ? k0 : k1), j & 63));
+        const uint32_t ij = (uint32_t)__builtin_amdgcn_readlane((int)(lo ? i0 : i1), j & 63);
+        r0 += kid_less(kj, ij, k0, i0);
+        r1 += kid_less(kj, ij, k1, i1);
+      }
+      if (lane < m) {
+        w.skey[r0] = k0;
+        w.sid[r0] = (int)i0;
+      }
+      if (lane + 64 < m) {
+        w.skey[r1] = k1;
+        w.sid[r1] = (int)i1;
       }
       WSYNC();
-      // rank sort, staged in A[R, 2R) (the capacity holds R more items)
-      for (int b = 0; b < R; b += 64) {
-        const int p = b + lane;
-        if (p < R) {
-          const uint2 v = A[p];
-          const float key = __uint_as_float(v.x);
-          int rank = 0;
-          for (int j = 0; j < R; ++j) {
-            const uint2 o = A[j];
-            const float kj = __uint_as_float(o.x);
-            rank += (kj < key) || (kj == key && o.y < v.y);
-          }
-          A[R + rank] = v;
-        }
+      // merge in batches of 64 (later batches sort after earlier ones)
+      for (int b = 0; b < m; b += 64) {
+        const int j = b + lane;
+        const bool in = j < m;
+        const float bj = in ? w.skey[j] : 0.f;
+        const uint32_t ij = in ? (uint32_t)w.sid[j] : 0u;
+        const int Lj = in ? lower_count_kid(A, n, bj, ij) : 0x7fffffff;
+        const int cnt = min(64, m - b);
+        const int L0 = __shfl(Lj, 0);
+        place_items(A, n, C, lane, cnt, Lj, make_uint2(__float_as_uint(bj), ij));
+        n = min(C, n + cnt);
+        if (L0 < ce) ce = L0;
         WSYNC();
       }
-      for (int p = lane; p < R; p += 64) A[p] = A[R + p];
-      WSYNC();
-      n = R;
     }
+    WSYNC();
+    ce = first_unused(A, ce, n, lane);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   }
-
-  // report the first min(k, n) items, ties by internal index
-  const int kk = min(k, n);
-  const int64_t row = (int64_t)qi * a.out_stride;
-  for (int b = 0; b < k; b += 64) {
-    const int p = b + lane;
-    if (p < kk) {
-      const uint2 v = A[p];
-      const float key = __uint_as_float(v.x);
-      const uint32_t id = v.y & ~kUsedBit;
-      const int start = lower_count(A, kk, key);
-      int rank = start;
-      for (int j = start; j < kk && akey(A, j) == key; ++j)
-        rank += ((A[j].y & ~kUsedBit) < id) ? 1 : 0;
-      a.out_ids[row + rank] = ix.ext_id[id];
-      a.out_dists[row + rank] = key;
-      if (a.out_internal) a.out_internal[row + rank] = (int32_t)id;
-    } else if (p < k) {
-      a.out_ids[row + p] = -1;
-      a.out_dists[row + p] = 0.f;
-      if (a.out_internal) a.out_internal[row + p] = -1;
-    }
-  }
-  if (lane == 0) {
-    a.out_counts[qi] = kk;
-    if (a.stats) {
-      atomicAdd(&a.stats->expansions, nexp);
-      atomicAdd(&a.stats->dist_level0, nd0);
-      atomicAdd(&a.stats->dist_upper, ndu);
-      atomicAdd(&a.stats->list_ints_read, nl);
-      atomicAdd(&a.stats->tie_paths, nties);
-    }
-  }
+  WSYNC();
+  const QueryCounters c{nexp, nd0, ndu, nl, 0};
+  finish_query<D, FUNC, ST>(a, w, n, qi, k, lane, h, c);
 }
 
 // ---------------------------------------------------------------------------
@@ -783,7 +1208,10 @@ __device__ void old_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, u
   }
 }
 
-template <int D, int FUNC, int ST>
+// WIDE = 1: the beam mode keeps a round's vector loads in flight together
+// (~240 VGPRs, one wave per SIMD) -- used when the LDS footprint already
+// limits residency to about a wave per SIMD; WIDE = 0 keeps occupancy high.
+template <int D, int FUNC, int ST, int WIDE>
 __global__ void __launch_bounds__(kWave* kWavesPerBlock)
     hnsw_search_kernel(SearchArgs a, int lds_wave_bytes) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -792,15 +1220,16 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock)
   const int h = lane & 1;
   const uint64_t ltm = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   const WaveLds w = carve(smem + wave * lds_wave_bytes, a.cap, a.ix.dstride);
-  const int64_t slot = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+  const int64_t slot = a.slot_base + (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
   uint8_t* vis = a.ix.visited + slot * a.ix.vstride;
   const int dim = a.ix.dim;
+  const int64_t nq = a.nq_dev ? *a.nq_dev : a.nq;
 
   for (;;) {
     int qi = 0;
     if (lane == 0) qi = atomicAdd(a.work, 1);
     qi = __shfl(qi, 0);
-    if (qi >= a.nq) break;
+    if (qi >= nq) break;
     const int k = a.k_of ? a.k_of[qi] : a.k;
     const int ef = a.ef_of ? a.ef_of[qi] : a.ef;
     const uint64_t t_start = __builtin_amdgcn_s_memrealtime();
@@ -848,6 +1277,8 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock)
     const bool use_old = a.algo == kAlgoOld || (a.algo == kAlgoHybrid && ef >= 1000);
     if (use_old)
       old_query<D, FUNC, ST>(a, w, vis, ep, qi, k, ef, lane, h, ltm, slot);
+    else if (a.beam > 0)
+      beam_query<D, FUNC, ST, WIDE>(a, w, vis, ep, qi, k, ef, lane, h, ltm);
     else
       v1merge_query<D, FUNC, ST>(a, w, vis, ep, qi, k, ef, lane, h, ltm);
     WSYNC();
@@ -858,59 +1289,3 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock)
 
 }  // namespace nmsamd
 
-// ---------------------------------------------------------------------------
-// Launch entry used by the host library (same translation unit keeps the
-// template instantiations private).
-namespace nmsamd {
-
-using KernelFn = void (*)(SearchArgs, int);
-
-template <int ST>
-static KernelFn pick_kernel(int dim, int func) {
-#define NMS_CASE(D)                                                   \
-  switch (func) {                                                     \
-    case kL2Sqr16: return hnsw_search_kernel<D, kL2Sqr16, ST>;        \
-    case kL2Sqr: return hnsw_search_kernel<D, kL2Sqr, ST>;            \
-    case kNormCos: return hnsw_search_kernel<D, kNormCos, ST>;        \
-    case kNegDot: return hnsw_search_kernel<D, kNegDot, ST>;          \
-    default: return nullptr;                                          \
-  }
-  switch (dim) {
-    case 96: NMS_CASE(96)
-    case 100: NMS_CASE(100)
-    case 128: NMS_CASE(128)
-    default: NMS_CASE(0)
-  }
-#undef NMS_CASE
-}
-
-hipError_t launch_search(const SearchArgs& a, int blocks, int waves_per_block, int lds_wave_bytes,
-                         hipStream_t stream) {
-  KernelFn fn = nullptr;
-  if (a.storage == kF32) fn = pick_kernel<kF32>(a.ix.dim, a.ix.func);
-  else if (a.storage == kBF16) fn = pick_kernel<kBF16>(a.ix.dim, a.ix.func);
-  else fn = pick_kernel<kFP8>(a.ix.dim, a.ix.func);
-  if (!fn) return hipErrorInvalidValue;
-  if (waves_per_block < 1 || waves_per_block > kWavesPerBlock) return hipErrorInvalidValue;
-  const size_t lds = (size_t)lds_wave_bytes * waves_per_block;
-  if (lds > 64 * 1024) {
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-  }
-  hipLaunchKernelGGL(fn, dim3(blocks), dim3(kWave * waves_per_block), lds, stream, a,
-                     lds_wave_bytes);
-  return hipGetLastError();
-}
-
-__global__ void fill_i32_kernel(int* p, int v, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
-
-hipError_t launch_fill_i32(int* p, int v, int n, hipStream_t stream) {
-  hipLaunchKernelGGL(fill_i32_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, p, v, n);
-  return hipGetLastError();
-}
-
-}  // namespace nmsamd

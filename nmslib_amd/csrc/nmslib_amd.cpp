@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <deque>
 #include <fstream>
 #include <memory>
 #include <mutex>
@@ -26,7 +27,7 @@
 
 namespace nmsamd {
 hipError_t launch_search(const SearchArgs& a, int blocks, int waves_per_block, int lds_wave_bytes,
-                         hipStream_t stream);
+                         int wide, hipStream_t stream);
 hipError_t launch_fill_i32(int* p, int v, int n, hipStream_t stream);
 hipError_t launch_convert_lowp(const float* vec, uint16_t* vec16, uint8_t* vec8, float* scale8,
                                int64_t n, int dim, int dstride, int storage, hipStream_t stream);
@@ -101,6 +102,7 @@ struct nmsamd_index {
   Algo algo = kHybrid;
   int storage = kF32;
   int rerank_mult = 0;
+  int beam = 0;  // 0: exact V1Merge; > 0: beam mode width
   bool collect = false;
   // file header (hnsw.cc:779-788)
   bool loaded = false;
@@ -127,6 +129,7 @@ struct nmsamd_index {
   DevBuf<int32_t> d_l0, d_upoff, d_up, d_ext;
   DevBuf<uint8_t> d_visited, d_epoch;
   int64_t vstride = 0;
+  int64_t vstride_server = 0;
   int slots = 0;
   // per-call buffers
   DevBuf<float> d_q, d_dists;
@@ -345,7 +348,7 @@ DevIndex dev_index(nmsamd_index* h) {
 }
 
 int lds_wave_bytes(int cap, int dstride) {
-  const int b = cap * 8 + dstride * 4 + 64 * 4 * 6;
+  const int b = ((cap + 1) & ~1) * 8 + dstride * 4 + 2048;  // + two 1 KB scratch regions
   return (b + 15) / 16 * 16;
 }
 
@@ -417,12 +420,16 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   h->last_nq = nq;
   a.storage = h->storage;
   a.rerank = rerank ? h->rerank_mult : 0;
+  a.beam = h->beam;
   a.algo = h->algo == kOld ? kAlgoOld : (h->algo == kV1Merge ? kAlgoV1Merge : kAlgoHybrid);
   a.old_heap_key = h->d_oldk.p;
   a.old_heap_id = h->d_oldi.p;
   a.old_heap_cap = heap_cap;
   hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
-  hip_check(launch_search(a, blocks, wpb, lds, s), "search kernel launch");
+  // beam mode keeps a round's gathers in flight together when the LDS
+  // footprint already holds residency to about one wave per SIMD
+  const int wide = (h->beam > 0 && wpb * per_cu <= 8) ? 1 : 0;
+  hip_check(launch_search(a, blocks, wpb, lds, wide, s), "search kernel launch");
   hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
   h->pending = true;
 }
@@ -711,6 +718,14 @@ int nmsamd_set_precision(nmsamd_index* h, int storage, int rerank_mult) {
   });
 }
 
+int nmsamd_set_beam(nmsamd_index* h, int beam) {
+  return guarded([&] {
+    if (!h) fail(NMSAMD_EINVAL, "null handle");
+    if (beam < 0 || beam > 4) fail(NMSAMD_EINVAL, "beam must be 0 (exact V1Merge) or 1..4");
+    h->beam = beam;
+  });
+}
+
 int nmsamd_last_stats(const nmsamd_index* hc, uint64_t* o, double* ms) {
   return guarded([&] {
     if (!hc) fail(NMSAMD_EINVAL, "null handle");
@@ -745,6 +760,246 @@ int nmsamd_collect_stats(nmsamd_index* h, int on) {
   return guarded([&] {
     if (!h) fail(NMSAMD_EINVAL, "null handle");
     h->collect = on != 0;
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Continuous-batching server (no reference counterpart; the reference serves
+// through query_server/, whose knnQueryBatch runs ParallelFor over Search,
+// QueryService_server.cpp:411-466).  A server owns `lanes` independent
+// launch slots over one loaded index: each lane has its own stream, device and
+// pinned host buffers and its own range of visited-mark slots, so batches
+// submitted to different lanes run concurrently on the device.  A caller
+// admits arriving requests into a batch whenever a lane is free; per-query k
+// and ef ride in the batch (the uneven persistent kernel).  Optionally the
+// fill + search launch of a lane is captured once as a hipGraph and replayed
+// (the query count is read on the device, so the graph's arguments stay fixed).
+namespace {
+
+struct ServerLane {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr, t0 = nullptr;
+  DevBuf<float> q, dists;
+  DevBuf<int32_t> k, ef, ids, counts, work;
+  DevBuf<int64_t> nq_dev;
+  float* hq = nullptr;  // pinned staging
+  int32_t *hk = nullptr, *hef = nullptr, *hids = nullptr, *hcounts = nullptr;
+  float* hdists = nullptr;
+  int64_t* hnq = nullptr;
+  int64_t nq = 0;
+  bool busy = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int64_t slot_base = 0;
+  int blocks = 0;
+};
+
+}  // namespace
+
+struct nmsamd_server {
+  nmsamd_index* h = nullptr;
+  int max_queries = 0, max_ef = 0, max_k = 0, cap = 0, lds = 0, wpb = 0, wide = 0;
+  int slots_per_lane = 0;
+  bool use_graph = false;
+  DevBuf<uint8_t> visited, epoch;
+  std::deque<ServerLane> lanes;  // lanes never move once created
+  ~nmsamd_server() {
+    for (auto& L : lanes) {
+      if (L.exec) (void)hipGraphExecDestroy(L.exec);
+      if (L.graph) (void)hipGraphDestroy(L.graph);
+      if (L.done) (void)hipEventDestroy(L.done);
+      if (L.t0) (void)hipEventDestroy(L.t0);
+      if (L.stream) (void)hipStreamDestroy(L.stream);
+      for (void* p : {(void*)L.hq, (void*)L.hk, (void*)L.hef, (void*)L.hids, (void*)L.hcounts,
+                      (void*)L.hdists, (void*)L.hnq})
+        if (p) (void)hipHostFree(p);
+    }
+  }
+};
+
+namespace {
+
+SearchArgs server_args(nmsamd_server* s, ServerLane& L) {
+  nmsamd_index* h = s->h;
+  SearchArgs a{};
+  a.ix = dev_index(h);
+  a.ix.visited = s->visited.p;
+  a.ix.epoch = s->epoch.p;
+  a.ix.vstride = h->vstride_server;
+  a.queries = L.q.p;
+  a.nq = 0;
+  a.nq_dev = L.nq_dev.p;
+  a.k_of = L.k.p;
+  a.ef_of = L.ef.p;
+  a.cap = s->cap;
+  a.out_stride = s->max_k;
+  a.out_ids = L.ids.p;
+  a.out_dists = L.dists.p;
+  a.out_counts = L.counts.p;
+  a.work = L.work.p;
+  a.err = L.work.p + 1;
+  a.storage = h->storage;
+  a.rerank = (h->storage != kF32 && h->rerank_mult > 0) ? h->rerank_mult : 0;
+  a.beam = h->beam;
+  a.algo = kAlgoV1Merge;
+  a.slot_base = L.slot_base;
+  return a;
+}
+
+void lane_launch(nmsamd_server* s, ServerLane& L) {
+  SearchArgs a = server_args(s, L);
+  hip_check(launch_fill_i32(L.work.p, 0, 2, L.stream), "fill");
+  hip_check(launch_search(a, L.blocks, s->wpb, s->lds, s->wide, L.stream), "search kernel launch");
+}
+
+}  // namespace
+
+extern "C" {
+
+int nmsamd_server_open(nmsamd_index* h, int lanes, int max_queries, int max_ef, int max_k,
+                       int use_graph, nmsamd_server** out) {
+  return guarded([&] {
+    if (!h || !out) fail(NMSAMD_EINVAL, "null argument");
+    if (!h->loaded) fail(NMSAMD_EINVAL, "Must call createIndex or loadIndex before this method");
+    if (lanes < 1 || lanes > 16 || max_queries < 1 || max_ef < 1 || max_k < 1)
+      fail(NMSAMD_EINVAL, "lanes in 1..16 and positive max_queries / max_ef / max_k required");
+    if (max_ef >= 1000 && h->algo != kV1Merge)
+      fail(NMSAMD_EINVAL, "the server runs V1Merge: keep ef below 1000 or set algoType=v1merge");
+    auto s = std::make_unique<nmsamd_server>();
+    s->h = h;
+    s->max_queries = max_queries;
+    s->max_ef = max_ef;
+    s->max_k = max_k;
+    s->cap = std::max(max_ef, max_k);
+    if (h->storage != kF32 && h->rerank_mult > 0)
+      s->cap = std::max(s->cap, std::max(max_ef, max_k) +
+                                    std::min(std::max(max_ef, max_k), h->rerank_mult * max_k));
+    s->lds = lds_wave_bytes(s->cap, h->dstride);
+    if ((size_t)s->lds > kLdsPerCu) fail(NMSAMD_EINVAL, "max_ef / max_k exceed the LDS of one wave");
+    ensure_lowp(h);
+    s->wpb = (int)std::max<size_t>(1, std::min<size_t>(kWavesPerBlock, kLdsPerCu / s->lds));
+    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, kLdsPerCu / ((size_t)s->lds * s->wpb)));
+    s->wide = (h->beam > 0 && s->wpb * per_cu <= 8) ? 1 : 0;
+    // the device's resident waves at this footprint, shared by the lanes
+    const int64_t total_blocks = (int64_t)h->cu_count * per_cu;
+    const int64_t blocks_per_lane = std::max<int64_t>(1, total_blocks / lanes);
+    s->slots_per_lane = (int)(blocks_per_lane * s->wpb);
+    const int64_t slots = (int64_t)s->slots_per_lane * lanes;
+    h->vstride_server = ((int64_t)h->n + 1 + 255) / 256 * 256;
+    s->visited.ensure((size_t)slots * h->vstride_server);
+    s->epoch.ensure((size_t)slots);
+    hip_check(hipMemset(s->visited.p, 0, (size_t)slots * h->vstride_server), "memset");
+    hip_check(hipMemset(s->epoch.p, 0, (size_t)slots), "memset");
+    s->use_graph = use_graph != 0;
+    for (int i = 0; i < lanes; ++i) s->lanes.emplace_back();
+    for (int i = 0; i < lanes; ++i) {
+      ServerLane& L = s->lanes[i];
+      hip_check(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking), "hipStreamCreate");
+      hip_check(hipEventCreateWithFlags(&L.done, hipEventDisableTiming), "hipEventCreate");
+      L.q.ensure((size_t)max_queries * h->dim);
+      L.dists.ensure((size_t)max_queries * max_k);
+      L.ids.ensure((size_t)max_queries * max_k);
+      L.counts.ensure((size_t)max_queries);
+      L.k.ensure((size_t)max_queries);
+      L.ef.ensure((size_t)max_queries);
+      L.work.ensure(2);
+      L.nq_dev.ensure(1);
+      hip_check(hipHostMalloc((void**)&L.hq, sizeof(float) * max_queries * h->dim), "hipHostMalloc");
+      hip_check(hipHostMalloc((void**)&L.hk, sizeof(int32_t) * max_queries), "hipHostMalloc");
+      hip_check(hipHostMalloc((void**)&L.hef, sizeof(int32_t) * max_queries), "hipHostMalloc");
+      hip_check(hipHostMalloc((void**)&L.hids, sizeof(int32_t) * max_queries * max_k), "hipHostMalloc");
+      hip_check(hipHostMalloc((void**)&L.hdists, sizeof(float) * max_queries * max_k), "hipHostMalloc");
+      hip_check(hipHostMalloc((void**)&L.hcounts, sizeof(int32_t) * max_queries), "hipHostMalloc");
+      hip_check(hipHostMalloc((void**)&L.hnq, sizeof(int64_t)), "hipHostMalloc");
+      L.slot_base = (int64_t)i * s->slots_per_lane;
+      L.blocks = (int)std::max<int64_t>(1, s->slots_per_lane / s->wpb);
+      if (s->use_graph) {
+        hip_check(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal), "capture");
+        lane_launch(s.get(), L);
+        hip_check(hipStreamEndCapture(L.stream, &L.graph), "capture");
+        hip_check(hipGraphInstantiate(&L.exec, L.graph, nullptr, nullptr, 0), "graph instantiate");
+      }
+    }
+    *out = s.release();
+  });
+}
+
+int nmsamd_server_submit(nmsamd_server* s, int lane, const float* q, int64_t nq, const int32_t* k,
+                         const int32_t* ef) {
+  return guarded([&] {
+    if (!s || lane < 0 || lane >= (int)s->lanes.size()) fail(NMSAMD_EINVAL, "bad server lane");
+    ServerLane& L = s->lanes[lane];
+    if (L.busy) fail(NMSAMD_EINVAL, "lane is busy");
+    if (nq < 1 || nq > s->max_queries) fail(NMSAMD_EINVAL, "batch size outside 1..max_queries");
+    for (int64_t i = 0; i < nq; ++i) {
+      if (k[i] < 1 || k[i] > s->max_k || ef[i] < 0 || ef[i] > s->max_ef)
+        fail(NMSAMD_EINVAL, "k or ef outside the server's limits");
+    }
+    const int dim = s->h->dim;
+    std::memcpy(L.hq, q, sizeof(float) * nq * dim);
+    std::memcpy(L.hk, k, sizeof(int32_t) * nq);
+    std::memcpy(L.hef, ef, sizeof(int32_t) * nq);
+    *L.hnq = nq;
+    hip_check(hipMemcpyAsync(L.q.p, L.hq, sizeof(float) * nq * dim, hipMemcpyHostToDevice, L.stream), "H2D");
+    hip_check(hipMemcpyAsync(L.k.p, L.hk, sizeof(int32_t) * nq, hipMemcpyHostToDevice, L.stream), "H2D");
+    hip_check(hipMemcpyAsync(L.ef.p, L.hef, sizeof(int32_t) * nq, hipMemcpyHostToDevice, L.stream), "H2D");
+    hip_check(hipMemcpyAsync(L.nq_dev.p, L.hnq, sizeof(int64_t), hipMemcpyHostToDevice, L.stream), "H2D");
+    if (L.exec) hip_check(hipGraphLaunch(L.exec, L.stream), "graph launch");
+    else lane_launch(s, L);
+    const size_t out = (size_t)nq * s->max_k;
+    hip_check(hipMemcpyAsync(L.hids, L.ids.p, sizeof(int32_t) * out, hipMemcpyDeviceToHost, L.stream), "D2H");
+    hip_check(hipMemcpyAsync(L.hdists, L.dists.p, sizeof(float) * out, hipMemcpyDeviceToHost, L.stream), "D2H");
+    hip_check(hipMemcpyAsync(L.hcounts, L.counts.p, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, L.stream), "D2H");
+    hip_check(hipEventRecord(L.done, L.stream), "hipEventRecord");
+    L.nq = nq;
+    L.busy = true;
+  });
+}
+
+int nmsamd_server_poll(nmsamd_server* s, int lane) {
+  if (!s || lane < 0 || lane >= (int)s->lanes.size()) {
+    g_err = "bad server lane";
+    return NMSAMD_EINVAL;
+  }
+  ServerLane& L = s->lanes[lane];
+  if (!L.busy) return 1;
+  const hipError_t e = hipEventQuery(L.done);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  g_err = std::string("server lane: ") + hipGetErrorString(e);
+  return NMSAMD_EDEVICE;
+}
+
+int nmsamd_server_fetch(nmsamd_server* s, int lane, int32_t* ids, float* dists, int32_t* counts) {
+  return guarded([&] {
+    if (!s || lane < 0 || lane >= (int)s->lanes.size()) fail(NMSAMD_EINVAL, "bad server lane");
+    ServerLane& L = s->lanes[lane];
+    if (!L.busy) fail(NMSAMD_EINVAL, "lane has no batch");
+    hip_check(hipEventSynchronize(L.done), "server lane");
+    const size_t out = (size_t)L.nq * s->max_k;
+    std::memcpy(ids, L.hids, sizeof(int32_t) * out);
+    std::memcpy(dists, L.hdists, sizeof(float) * out);
+    std::memcpy(counts, L.hcounts, sizeof(int32_t) * L.nq);
+    L.busy = false;
+  });
+}
+
+int nmsamd_server_info(const nmsamd_server* s, int64_t* o) {
+  return guarded([&] {
+    if (!s || !o) fail(NMSAMD_EINVAL, "null argument");
+    o[0] = (int64_t)s->lanes.size(); o[1] = s->max_queries; o[2] = s->max_ef; o[3] = s->max_k;
+    o[4] = s->slots_per_lane; o[5] = s->wpb; o[6] = s->use_graph ? 1 : 0; o[7] = s->wide;
+  });
+}
+
+int nmsamd_server_close(nmsamd_server* s) {
+  return guarded([&] {
+    if (!s) return;
+    for (auto& L : s->lanes)
+      if (L.stream) (void)hipStreamSynchronize(L.stream);
+    delete s;
   });
 }
 

@@ -1,0 +1,33 @@
+// Instantiations of the search kernels for fp8 vector storage.
+#include "hnsw_search_impl.h"
+
+namespace nmsamd {
+
+using KernelFn = void (*)(SearchArgs, int);
+
+KernelFn pick_kernel_fp8(int dim, int func, int wide) {
+#define NMS_CASE(D, W)                                                 \
+  switch (func) {                                                      \
+    case kL2Sqr16: return hnsw_search_kernel<D, kL2Sqr16, kFP8, W>;      \
+    case kL2Sqr: return hnsw_search_kernel<D, kL2Sqr, kFP8, W>;          \
+    case kNormCos: return hnsw_search_kernel<D, kNormCos, kFP8, W>;      \
+    case kNegDot: return hnsw_search_kernel<D, kNegDot, kFP8, W>;        \
+    default: return nullptr;                                           \
+  }
+#define NMS_DIMS(W)                \
+  switch (dim) {                   \
+    case 96: NMS_CASE(96, W)       \
+    case 100: NMS_CASE(100, W)     \
+    case 128: NMS_CASE(128, W)     \
+    default: NMS_CASE(0, W)        \
+  }
+  if (wide) {
+    NMS_DIMS(1)
+  } else {
+    NMS_DIMS(0)
+  }
+#undef NMS_DIMS
+#undef NMS_CASE
+}
+
+}  // namespace nmsamd

@@ -144,3 +144,90 @@ def test_gpu_wrong_space_for_index_is_rejected():
     name = "gmm96_nd"
     with pytest.raises(RuntimeError, match="does not match space l2"):
         load(name, F.index_path(name), "l2")
+
+
+def _exact_dist(func, q, x):
+    return np.float32(O.dist(func, q, x))
+
+
+@pytest.mark.parametrize("name", ["u128_l2", "dec100_cos", "gmm96_nd"])
+def test_gpu_beam1_equals_exact_without_ties(name):
+    # beam mode with one expansion per step is V1Merge with ties ordered by id;
+    # these fixtures have no tied distances, so results must match exactly
+    m = F.META[name]
+    idx = load(name, F.index_path(name), m["space"], 64)
+    q = F.results(name)["queries"][1:]  # skip the all-zero cosine query (all ties)
+    exact = idx.knnQueryBatchArrays(q, 10)
+    idx.setBeam(1)
+    beam = idx.knnQueryBatchArrays(q, 10)
+    for x, y in zip(exact, beam):
+        np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("name", ["u128_l2", "hn128_l2", "dec100_cos", "gmm96_nd"])
+@pytest.mark.parametrize("beam", [2, 4])
+def test_gpu_beam_mode_quality(name, beam):
+    m = F.META[name]
+    idx = load(name, F.index_path(name), m["space"], 100)
+    res = F.results(name)
+    q = res["queries"]
+    ex_ids, ex_d, _ = idx.knnQueryBatchArrays(q, 10)
+    idx.setBeam(beam)
+    ids, dists, counts = idx.knnQueryBatchArrays(q, 10)
+    assert np.all(counts == 10)
+    x = F.base_vectors(name)
+    oracle = O.OracleIndex(F.index_path(name))
+    blk = oracle.level0_block().reshape(oracle.n, oracle.elem_bytes)
+    vec = blk[:, 16:16 + 4 * oracle.dim].copy().view(np.float32)  # as stored (normalised for cosine)
+    for i in range(len(q)):
+        row = ids[i]
+        assert len(set(row.tolist())) == 10
+        assert np.all(np.diff(dists[i]) >= 0)
+        qq = O.normalize(q[i]) if m["space"] == "cosinesimil" else q[i]
+        for j in (0, 9):
+            assert dists[i, j] == _exact_dist(oracle.func, qq, vec[row[j]])
+    # recall against brute force stays within 0.3 points of exact mode (or above)
+    gt, gd = O.brute_force(x, q, 100, m["space"])
+    r_exact = F.recall_tie_aware(ex_ids, gt, gd, 10, rel=1e-5)
+    r_beam = F.recall_tie_aware(ids, gt, gd, 10, rel=1e-5)
+    assert r_beam >= r_exact - 0.003, (r_beam, r_exact)
+
+
+def test_gpu_bf16_on_integer_data_is_exact():
+    # coordinates 0..255 are exact in BF16 and every partial sum is an exact
+    # integer below 2^24, so BF16 traversal reproduces the FP32 results
+    name = "hn128_l2"
+    idx = load(name, F.index_path(name), "l2", 100)
+    q = F.results(name)["queries"]
+    ref = idx.knnQueryBatchArrays(q, 10)
+    idx.setPrecision("bf16", 0)
+    got = idx.knnQueryBatchArrays(q, 10)
+    for x, y in zip(ref, got):
+        np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("storage", ["bf16", "fp8"])
+@pytest.mark.parametrize("name", ["u128_l2", "dec100_cos", "gmm96_nd"])
+def test_gpu_reduced_precision_with_rerank(name, storage):
+    m = F.META[name]
+    idx = load(name, F.index_path(name), m["space"], 100)
+    res = F.results(name)
+    q = res["queries"]
+    ex_ids, _, _ = idx.knnQueryBatchArrays(q, 10)
+    idx.setPrecision(storage, 4)
+    ids, dists, counts = idx.knnQueryBatchArrays(q, 10)
+    oracle = O.OracleIndex(F.index_path(name))
+    blk = oracle.level0_block().reshape(oracle.n, oracle.elem_bytes)
+    vec = blk[:, 16:16 + 4 * oracle.dim].copy().view(np.float32)
+    # reranked distances are the reference's FP32 distances, bit for bit
+    for i in range(len(q)):
+        qq = O.normalize(q[i]) if m["space"] == "cosinesimil" else q[i]
+        for j in range(counts[i]):
+            assert dists[i, j] == _exact_dist(oracle.func, qq, vec[ids[i, j]])
+    x = F.base_vectors(name)
+    gt, gd = O.brute_force(x, q, 100, m["space"])
+    r_exact = F.recall_tie_aware(ex_ids, gt, gd, 10, rel=1e-5)
+    r_low = F.recall_tie_aware(ids, gt, gd, 10, rel=1e-5)
+    tol = 0.01 if storage == "bf16" else 0.05
+    print(f"{name} {storage}: recall exact {r_exact:.4f} reduced+rerank {r_low:.4f}")
+    assert r_low >= r_exact - tol
