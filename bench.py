@@ -104,8 +104,10 @@ def cpu_baseline(path, q, gids, gd, ef, algo, gpu_ids, gpu_dists):
     el = time.time() - t
     # exactly tied distances come back in Object-address order from the
     # reference (knnqueue.h:73-74): compare rows with ties ordered by id
-    same_rows = int(sum(np.array_equal(a, b) and np.array_equal(c, d) for a, c, b, d in
-                        zip(*_canon(ids, dists), *_canon(gpu_ids[:ns], gpu_dists[:ns]))))
+    same_rows = -1
+    if gpu_ids is not None:
+        same_rows = int(sum(np.array_equal(a, b) and np.array_equal(c, d) for a, c, b, d in
+                            zip(*_canon(ids, dists), *_canon(gpu_ids[:ns], gpu_dists[:ns]))))
     rec = recall_at_k(ids, gids[:ns], gd[:ns])
     del run
     return {"value": round(ns / el, 2), "unit": "queries/s", "cores": threads, "kind": kind,
@@ -114,54 +116,65 @@ def cpu_baseline(path, q, gids, gd, ef, algo, gpu_ids, gpu_dists):
                       f"GPU's (ties ordered by id): {same_rows}/{ns}"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--ef", type=int, default=0, help="fixed efSearch (0: smallest of the ladder "
-                    "reaching recall@10 >= 0.95)")
-    ap.add_argument("--algo", default="v1merge")
-    ap.add_argument("--storage", default="fp32", choices=["fp32", "bf16", "fp8"])
-    ap.add_argument("--rerank", type=int, default=0, help="FP32 rerank width as a multiple of k")
-    ap.add_argument("--beam", type=int, default=0, help="0: exact V1Merge; 1..4: beam mode")
-    ap.add_argument("--nq", type=int, default=10000)
-    ap.add_argument("--sweep", action="store_true", help="also print the recall/QPS curve")
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the query path "
-              f"shards by replicating the index per GPU, see DESIGN.md", file=sys.stderr)
-        sys.exit(2)
+def _measured_traffic(path, form, ef):
+    """HBM bytes per launch of `form` at `ef`, from the committed summary of
+    the FETCH_SIZE / WRITE_SIZE counter runs (a line
+    'traffic <form> ef=<ef> bytes=<n>'), or None when not measured."""
+    import re
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(rf"^traffic {re.escape(form)} ef={ef} bytes=(\d+)", text, re.M)
+    return int(m.group(1)) if m else None
 
+
+def serve_bench(idx, q, forms, rates, duration_s):
+    """The serving loop over the same resident index: Poisson arrivals, each
+    request 1..256 queries with k in {1, 10, 100} and ef in 50..800, admitted
+    into free lanes as they arrive (continuous batching)."""
+    from nmslib_amd import serving
+
+    out = []
+    for name, storage, rerank, beam in forms:
+        idx.setPrecision(storage, rerank)
+        idx.setBeam(beam)
+        idx.setQueryTimeParams({"algoType": "v1merge"})
+        srv = serving.Server(idx, lanes=4, max_queries=2048, max_ef=800, max_k=100, use_graph=True)
+        for rate in rates:
+            reqs = serving.make_requests(len(q), rate, duration_s, seed=int(rate))
+            st = serving.run_poisson(srv, q, reqs)
+            st.update({"form": name, "offered_qps": rate})
+            out.append({k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()})
+            log(f"[bench] serve {name} offered={rate} -> {out[-1]}")
+        srv.close()
+    return out
+
+
+FORMS = {
+    # name: (storage, rerank multiple of k, beam width)
+    "fp32-exact": ("fp32", 0, 0),
+    "fp32-beam4": ("fp32", 0, 4),
+    "bf16-beam4-rerank": ("bf16", 2, 4),
+}
+
+
+def measure(idx, q, gids, gd, form, args, dev, log_prefix):
+    """Ladder to recall@10 >= 0.95, then a counters run and the timed steps."""
     import torch
 
-    import nmslib_amd as nmslib
-    from nmslib_amd import datasets as D
-
-    tmpdir = tempfile.mkdtemp(prefix="nmsamd_bench_")
-    path, _ = prepare_index(tmpdir)
-    q = D.queries(WORKLOAD, args.nq)
-    gt = np.load(GT)
-    gids, gd = gt["ids"][:args.nq], gt["dists"][:args.nq]
-    k = 10
-
-    idx = nmslib.init(method="hnsw", space="l2")
-    t = time.time()
-    idx.loadIndex(path)
-    log(f"[bench] loadIndex {time.time() - t:.1f}s {idx.info()}")
-    if args.storage != "fp32":
-        idx.setPrecision(args.storage, args.rerank)
-    idx.setBeam(args.beam)
-    dev = torch.device("cuda:0")
+    storage, rerank, beam = FORMS[form]
+    idx.setPrecision(storage, rerank)
+    idx.setBeam(beam)
+    nq, k = len(q), 10
     dq = torch.from_numpy(q).to(dev)
-    dids = torch.empty((args.nq, k), dtype=torch.int32, device=dev)
-    dd = torch.empty((args.nq, k), dtype=torch.float32, device=dev)
-    dc = torch.empty((args.nq,), dtype=torch.int32, device=dev)
+    dids = torch.empty((nq, k), dtype=torch.int32, device=dev)
+    dd = torch.empty((nq, k), dtype=torch.float32, device=dev)
+    dc = torch.empty((nq,), dtype=torch.int32, device=dev)
     stream = torch.cuda.current_stream(dev)
 
     def step():
-        idx.knnQueryBatchDevice(dq.data_ptr(), args.nq, k, dids.data_ptr(), dd.data_ptr(),
+        idx.knnQueryBatchDevice(dq.data_ptr(), nq, k, dids.data_ptr(), dd.data_ptr(),
                                 dc.data_ptr(), stream.cuda_stream)
 
     def run_at(ef):
@@ -177,13 +190,13 @@ def main():
     if args.sweep:
         for ef in SWEEP:
             r, el = run_at(ef)
-            curve.append({"ef": ef, "recall": round(r, 4), "qps": round(args.nq / el, 1)})
-            log(f"[bench] sweep ef={ef} recall={r:.4f} qps={args.nq / el:.0f}")
+            curve.append({"ef": ef, "recall": round(r, 4), "qps": round(nq / el, 1)})
+            log(f"{log_prefix} sweep ef={ef} recall={r:.4f} qps={nq / el:.0f}")
     ef = args.ef
     if not ef:
         for cand in EF_LADDER:
             r, el = run_at(cand)
-            log(f"[bench] ladder ef={cand} recall={r:.4f} ({args.nq / el:.0f} q/s)")
+            log(f"{log_prefix} ladder ef={cand} recall={r:.4f} ({nq / el:.0f} q/s)")
             if r >= 0.95:
                 ef = cand
                 break
@@ -196,7 +209,7 @@ def main():
     step()
     torch.cuda.synchronize()
     st = idx.lastStats()
-    lat = idx.lastQueryTimesUs(args.nq)
+    lat = idx.lastQueryTimesUs(nq)
     idx.collectStats(False)
     ids_np = dids.cpu().numpy()
     dists_np = dd.cpu().numpy()
@@ -213,53 +226,149 @@ def main():
         kernel_ms.append(idx.lastStats()["kernel_ms"])
     torch.cuda.synchronize()
     elapsed = time.time() - t0
-    ms_per_step = elapsed * 1000 / args.steps
-    qps = args.nq * args.steps / elapsed
 
     # algorithmic bytes of one launch: adjacency ints read, every distance
-    # evaluation's vector, the queries in and the results out
+    # evaluation's vector, the rerank's FP32 vectors, queries in, results out
     dim = q.shape[1]
-    vec_bytes = {"fp32": 4, "bf16": 2, "fp8": 1}[args.storage] * dim + (4 if args.storage == "fp8" else 0)
+    vec_bytes = {"fp32": 4, "bf16": 2, "fp8": 1}[storage] * dim + (4 if storage == "fp8" else 0)
+    rerank_bytes = nq * min(ef, rerank * k) * dim * 4 if rerank else 0
     alg_bytes = (st["list_ints_read"] * 4 + (st["dist_level0"] + st["dist_upper"]) * vec_bytes
-                 + args.nq * dim * 4 + args.nq * (k * 8 + 4))
+                 + rerank_bytes + nq * dim * 4 + nq * (k * 8 + 4))
     avg_kernel_s = float(np.mean(kernel_ms)) / 1000
-    achieved = alg_bytes / avg_kernel_s / 1e9
+    res = {
+        "form": form, "efSearch": ef, "recall_at_10": round(recall, 4),
+        "qps": round(nq * args.steps / elapsed, 1),
+        "ms_per_step": round(elapsed * 1000 / args.steps, 3),
+        "avg_kernel_ms": round(avg_kernel_s * 1000, 3),
+        "latency_us_p50": round(float(np.percentile(lat, 50)), 1),
+        "latency_us_p99": round(float(np.percentile(lat, 99)), 1),
+        "per_query": {"expansions": round(st["expansions"] / nq, 1),
+                      "dist_evals": round((st["dist_level0"] + st["dist_upper"]) / nq, 1),
+                      "tie_path_expansions": round(st["tie_paths"] / nq, 1)},
+        "alg_bytes_per_launch": int(alg_bytes),
+        "achieved_gbs": round(alg_bytes / avg_kernel_s / 1e9, 1),
+        "curve": curve,
+    }
+    log(f"{log_prefix} {form}: {res}")
+    return res, ids_np, dists_np
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ef", type=int, default=0, help="fixed efSearch (0: smallest of the ladder "
+                    "reaching recall@10 >= 0.95)")
+    ap.add_argument("--algo", default="v1merge")
+    ap.add_argument("--forms", default="fp32-exact,bf16-beam4-rerank",
+                    help=f"comma list of {sorted(FORMS)}; the headline is the fastest")
+    ap.add_argument("--nq", type=int, default=10000)
+    ap.add_argument("--sweep", action="store_true", help="also record the recall/QPS curve")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve", default="20000,100000",
+                    help="offered query rates (q/s) of the serving run, '' to skip")
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--pmc-pass", action="store_true",
+                    help="only a 1 GiB copy and one search launch (for rocprofv3 --pmc runs)")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "hnsw_search_kernel_halfnormal1m_traffic.md"),
+                    help="HBM bytes per launch measured from FETCH_SIZE/WRITE_SIZE runs")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the query path "
+              f"would shard by replicating the index per GPU (one process each)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import nmslib_amd as nmslib
+    from nmslib_amd import datasets as D
+
+    tmpdir = tempfile.mkdtemp(prefix="nmsamd_bench_")
+    path, _ = prepare_index(tmpdir)
+    q = D.queries(WORKLOAD, args.nq)
+    gt = np.load(GT)
+    gids, gd = gt["ids"][:args.nq], gt["dists"][:args.nq]
+
+    idx = nmslib.init(method="hnsw", space="l2")
+    t = time.time()
+    idx.loadIndex(path)
+    log(f"[bench] loadIndex {time.time() - t:.1f}s {idx.info()}")
+    dev = torch.device("cuda:0")
+
+    if args.pmc_pass:
+        # short fixed pass for hardware-counter runs (rocprofv3 --pmc): a plain
+        # copy of known size (the counters' scale), then one search launch
+        nbytes = 1 << 30
+        src = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        form = args.forms.split(",")[-1]
+        storage, rerank, beam = FORMS[form]
+        idx.setPrecision(storage, rerank)
+        idx.setBeam(beam)
+        idx.setQueryTimeParams({"efSearch": args.ef or 4736, "algoType": args.algo})
+        ids, dists, counts = idx.knnQueryBatchArrays(q, 10)
+        log(f"[bench] pmc pass: copy {nbytes} B, one {form} launch, recall "
+            f"{recall_at_k(ids, gids, gd):.4f}")
+        return
+
+    forms = [f for f in args.forms.split(",") if f]
+    results, exact_out = {}, None
+    for form in forms:
+        res, ids_np, dists_np = measure(idx, q, gids, gd, form, args, dev, "[bench]")
+        results[form] = res
+        if form == "fp32-exact":
+            exact_out = (res["efSearch"], ids_np, dists_np)
+    head = max(results.values(), key=lambda r: r["qps"])
+
+    serve = None
+    if args.serve:
+        rates = [float(r) for r in args.serve.split(",") if r]
+        serve = serve_bench(idx, q, [("fp32 exact V1Merge", "fp32", 0, 0),
+                                     ("bf16 beam4 + fp32 rerank x2", "bf16", 2, 4)],
+                            rates, args.serve_seconds)
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(path, q, gids, gd, ef, args.algo, ids_np, dists_np)
+        if exact_out is not None:
+            ef_cpu, ids_cmp, d_cmp = exact_out
+        else:
+            ef_cpu, ids_cmp, d_cmp = head["efSearch"], None, None
+        cpu = cpu_baseline(path, q, gids, gd, ef_cpu, args.algo, ids_cmp, d_cmp)
         log(f"[bench] cpu baseline {cpu}")
 
     out = {
         "metric": "QPS at recall@10>=0.95 (1M x 128 L2, 1 GPU) + p50/p99 latency",
-        "value": round(qps, 1),
+        "value": head["qps"],
         "unit": "queries/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.storage == "fp32" else args.storage,
+        "dtype": "bf16" if head["form"].startswith("bf16") else "f32",
         "data": "synthetic (seeded generator; reference-built index)",
         "config": {"workload": "1,000,000 x 128 l2 half-normal u8-valued, M=16 efC=500, "
                                "10,000 queries, k=10",
-                   "efSearch": ef, "algoType": args.algo, "storage": args.storage,
-                   "rerank": args.rerank, "mode": "exact V1Merge" if not args.beam else f"beam {args.beam}",
-                   "recall_at_10": round(recall, 4),
-                   "latency_us_p50": round(float(np.percentile(lat, 50)), 1),
-                   "latency_us_p99": round(float(np.percentile(lat, 99)), 1),
+                   "headline_form": head["form"], "efSearch": head["efSearch"],
+                   "algoType": args.algo, "recall_at_10": head["recall_at_10"],
+                   "latency_us_p50": head["latency_us_p50"],
+                   "latency_us_p99": head["latency_us_p99"],
                    "latency_kind": "per-query device search time inside the batch",
-                   "per_query": {"expansions": st["expansions"] / args.nq,
-                                 "dist_evals": (st["dist_level0"] + st["dist_upper"]) / args.nq,
-                                 "tie_path_expansions": st["tie_paths"] / args.nq},
-                   "curve": curve},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "hnsw_search_kernel", "avg_kernel_ms": round(avg_kernel_s * 1000, 3),
-                     "alg_bytes_per_launch": int(alg_bytes)},
+                   "forms": results},
+        "roofline": {"bound": "hbm", "achieved": head["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(head["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": _measured_traffic(args.traffic, head["form"], head["efSearch"]),
+                     "kernel": "hnsw_search_kernel",
+                     "avg_kernel_ms": head["avg_kernel_ms"],
+                     "alg_bytes_per_launch": head["alg_bytes_per_launch"]},
         "cpu_baseline": cpu,
+        "serving": serve,
     }
     print(json.dumps(out), flush=True)
 

@@ -1,0 +1,56 @@
+"""The continuous-batching server: results are the same alone and in the
+mix, with and without hipGraph replay, under a Poisson request stream."""
+import numpy as np
+import pytest
+
+from nmslib_amd import serving
+from tests import fixtures as F
+from tests.gpu_util import load
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+@pytest.mark.parametrize("beam", [0, 4])
+def test_server_mix_equals_alone(use_graph, beam):
+    name = "u128_l2"
+    idx = load(name, F.index_path(name), "l2")
+    idx.setBeam(beam)
+    pool = F.results(name)["queries"]
+    srv = serving.Server(idx, lanes=3, max_queries=512, max_ef=400, max_k=100,
+                         use_graph=use_graph)
+    info = srv.info()
+    assert info["lanes"] == 3 and info["graph"] == int(use_graph)
+    reqs = serving.make_requests(len(pool), rate_qps=40000, duration_s=0.25, seed=7,
+                                 size_range=(1, 64), ks=(1, 10, 100), ef_range=(16, 400))
+    assert len(reqs) > 50
+    stats = serving.run_poisson(srv, pool, reqs, keep_results=True)
+    assert stats["requests"] == len(reqs)
+    print(stats)
+    # each request alone, through the plain batch entry point
+    for r in reqs[::7]:
+        q = pool[r.rows]
+        ids, dists, counts = idx.knnQueryBatchUneven(q, np.full(len(q), r.k, np.int32),
+                                                     np.full(len(q), r.ef, np.int32))
+        gi, gd, gc = r.result
+        np.testing.assert_array_equal(gc, counts)
+        np.testing.assert_array_equal(gi, ids[:, :r.k])
+        np.testing.assert_array_equal(gd, dists[:, :r.k])
+    srv.close()
+
+
+def test_server_rejects_out_of_budget_batches():
+    name = "u128_l2"
+    idx = load(name, F.index_path(name), "l2")
+    srv = serving.Server(idx, lanes=1, max_queries=8, max_ef=64, max_k=10, use_graph=False)
+    q = F.results(name)["queries"][:9]
+    with pytest.raises(ValueError):
+        srv.submit(0, q, np.full(9, 10, np.int32), np.full(9, 32, np.int32))
+    with pytest.raises(ValueError):
+        srv.submit(0, q[:2], np.full(2, 10, np.int32), np.full(2, 65, np.int32))
+    srv.submit(0, q[:2], np.full(2, 10, np.int32), np.full(2, 64, np.int32))
+    with pytest.raises(ValueError):  # lane busy
+        srv.submit(0, q[:2], np.full(2, 10, np.int32), np.full(2, 64, np.int32))
+    ids, dists, counts = srv.fetch(0)
+    assert counts.tolist() == [10, 10]
+    srv.close()

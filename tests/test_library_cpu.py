@@ -23,7 +23,8 @@ def test_library_exports_every_declared_symbol():
     assert len(syms) >= 14
     for s in syms:
         assert hasattr(lib, s), s
-    assert set(syms) == set(nmslib.SIGNATURES)
+    from nmslib_amd import serving
+    assert set(syms) == set(nmslib.SIGNATURES) | set(serving.SERVER_SIGNATURES)
 
 
 def test_init_spaces_and_methods():
