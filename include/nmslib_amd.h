@@ -115,6 +115,15 @@ int nmsamd_collect_stats(nmsamd_index* h, int on);
  * reference times queries on the host, experiments.h:186-194). */
 int nmsamd_last_query_us(const nmsamd_index* h, float* out_us, int64_t nq);
 
+/* Exact k-NN over the loaded index's FP32 vectors on MFMA: the device
+ * counterpart of the reference's seq_search method (method/seqsearch.cc:
+ * 143-177) and of its gold standard (gold_standard.h:151-175: ranked by
+ * (distance, id)).  Distances as the HNSW search reports them (squared L2,
+ * 1 - cos, -dot), computed as |x|^2 + |q|^2 - 2 q.x for l2; dims <= 128,
+ * 1 <= k <= 128.  Output [nq][k] rows, counts[nq]. */
+int nmsamd_brute_force(nmsamd_index* h, const float* q, int64_t nq, int64_t dim, int32_t k,
+                       int32_t* ids, float* dists, int32_t* counts);
+
 /* ---- continuous-batching server ------------------------------------------
  * No reference counterpart in the library; it stands where the reference's
  * Thrift query server (query_server/cpp_client_server/QueryService_server.cpp:

@@ -66,6 +66,7 @@ SIGNATURES = {
     "nmsamd_collect_stats": (C.c_int, [_P, C.c_int]),
     "nmsamd_last_query_us": (C.c_int, [_P, _P, _I64]),
     "nmsamd_set_beam": (C.c_int, [_P, C.c_int]),
+    "nmsamd_brute_force": (C.c_int, [_P, _P, _I64, _I64, _I32, _P, _P, _P]),
 }
 
 
@@ -227,6 +228,18 @@ class FloatIndex:
     def knnQuery(self, vector, k: int = 10):
         ids, dists = self.knnQueryBatch(np.asarray(vector, np.float32).reshape(1, -1), k)[0]
         return ids, dists
+
+    def bruteForce(self, queries, k: int = 10):
+        """Exact k-NN on the device (MFMA), the counterpart of the reference's
+        seq_search: (ids[nq, k], dists[nq, k], counts[nq]) by (distance, id)."""
+        q = _as_queries(queries, self._dim)
+        nq = q.shape[0]
+        ids = np.empty((nq, k), np.int32)
+        dists = np.empty((nq, k), np.float32)
+        counts = np.empty(nq, np.int32)
+        _check(load_library().nmsamd_brute_force(self._h, _ptr(q), nq, q.shape[1], int(k),
+                                                 _ptr(ids), _ptr(dists), _ptr(counts)))
+        return ids, dists, counts
 
     def knnQueryBatchUneven(self, queries, ks, efs):
         """One persistent launch with per-query k and ef."""

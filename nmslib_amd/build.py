@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libnmslib_amd.so")
 SOURCES = ["search_f32.hip", "search_bf16.hip", "search_fp8.hip", "launch.hip",
-           "lowp_convert.hip", "nmslib_amd.cpp"]
+           "lowp_convert.hip", "bruteforce.hip", "nmslib_amd.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -82,6 +82,7 @@ struct SearchArgs {
   int rerank;                // > 0: exact FP32 rerank of the best rerank*k candidates
   int beam;                  // > 0: beam mode, expansions per step (extension)
   int64_t slot_base;         // first visited-mark slot of this launch
+  const int32_t* order;      // optional: the order in which waves take queries
   const int64_t* nq_dev;     // optional: the query count read on the device
                              // (a replayed graph keeps its kernel arguments)
   int algo;                  // AlgoMode

@@ -1230,6 +1230,7 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock)
     if (lane == 0) qi = atomicAdd(a.work, 1);
     qi = __shfl(qi, 0);
     if (qi >= nq) break;
+    if (a.order) qi = a.order[qi];  // uneven batches: largest ef first
     const int k = a.k_of ? a.k_of[qi] : a.k;
     const int ef = a.ef_of ? a.ef_of[qi] : a.ef;
     const uint64_t t_start = __builtin_amdgcn_s_memrealtime();

@@ -138,6 +138,10 @@ struct nmsamd_index {
   DevBuf<uint32_t> d_qticks;
   DevBuf<float> d_oldk;
   DevBuf<int32_t> d_oldi;
+  DevBuf<int32_t> d_order;  // uneven batches: query order
+  const int32_t* order_dev = nullptr;
+  DevBuf<float> d_xnorm, d_bq, d_bpd;  // brute force: |x|^2, padded queries, partial lists
+  DevBuf<int32_t> d_bpi;
   bool pending = false;
   int64_t last_nq = 0;
   SearchStats last_stats{};
@@ -404,6 +408,7 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   a.nq = nq;
   a.k_of = k_of;
   a.ef_of = ef_of;
+  a.order = h->order_dev;
   a.k = k;
   a.ef = ef;
   a.cap = cap;
@@ -685,11 +690,21 @@ int nmsamd_knn_query_batch_uneven(nmsamd_index* h, const float* q, int64_t nq, i
     h->d_q.upload(q, (size_t)nq * dim, s);
     h->d_k.upload(k, (size_t)nq, s);
     h->d_ef.upload(ef, (size_t)nq, s);
+    // longest-first order: waves pull the largest max(ef, k) queries first so
+    // the persistent grid's tail is short
+    std::vector<int32_t> order((size_t)nq);
+    for (int64_t i = 0; i < nq; ++i) order[i] = (int32_t)i;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+      return std::max(ef[x], k[x]) > std::max(ef[y], k[y]);
+    });
+    h->d_order.upload(order.data(), order.size(), s);
     h->d_ids.ensure((size_t)nq * max_k);
     h->d_dists.ensure((size_t)nq * max_k);
     h->d_counts.ensure((size_t)nq);
+    h->order_dev = h->d_order.p;
     run_search(h, h->d_q.p, nq, 0, 0, h->d_k.p, h->d_ef.p, max_k, mef, h->d_ids.p, h->d_dists.p,
                h->d_counts.p, s);
+    h->order_dev = nullptr;
     hip_check(hipMemcpyAsync(ids, h->d_ids.p, sizeof(int32_t) * nq * max_k, hipMemcpyDeviceToHost, s), "D2H");
     hip_check(hipMemcpyAsync(dists, h->d_dists.p, sizeof(float) * nq * max_k, hipMemcpyDeviceToHost, s), "D2H");
     hip_check(hipMemcpyAsync(counts, h->d_counts.p, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, s), "D2H");
@@ -1004,3 +1019,70 @@ int nmsamd_server_close(nmsamd_server* s) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Exact k-NN by brute force on MFMA (the device seq_search,
+// similarity_search/src/method/seqsearch.cc:143-177).
+namespace nmsamd {
+hipError_t launch_brute_norms(const float* vec, float* out, int64_t n, int dstride,
+                              hipStream_t stream);
+hipError_t launch_brute(const float* vec, const float* xnorm, const float* queries, int64_t n,
+                        int64_t nq, int dstride, int func, int k, int nchunks, float* part_d,
+                        int32_t* part_i, const int32_t* ext_id, int32_t* ids, float* dists,
+                        int32_t* counts, hipStream_t stream);
+}  // namespace nmsamd
+
+extern "C" int nmsamd_brute_force(nmsamd_index* h, const float* q, int64_t nq, int64_t dim,
+                                  int32_t k, int32_t* ids, float* dists, int32_t* counts) {
+  return guarded([&] {
+    if (!h) fail(NMSAMD_EINVAL, "null handle");
+    std::lock_guard<std::mutex> g(h->mu);
+    if (!h->loaded) fail(NMSAMD_EINVAL, "Must call createIndex or loadIndex before this method");
+    if (dim != h->dim) fail(NMSAMD_EINVAL, "query dimension does not match the index");
+    if (h->dstride > 128) fail(NMSAMD_EINVAL, "brute force supports dimensions up to 128");
+    if (k < 1 || k > 128) fail(NMSAMD_EINVAL, "brute force k must be in 1..128");
+    if (nq <= 0) return;
+    hipStream_t s = h->stream;
+    if (!h->d_xnorm.p) {
+      h->d_xnorm.ensure((size_t)std::max<int64_t>(h->n, 1));
+      hip_check(launch_brute_norms(h->d_vec.p, h->d_xnorm.p, h->n, h->dstride, s), "norms");
+    }
+    // queries padded to the index row stride, normalised for cosine as the
+    // index vectors are (hnsw.h:486-497)
+    std::vector<float> qp((size_t)nq * h->dstride, 0.f);
+    for (int64_t i = 0; i < nq; ++i) {
+      float* row = &qp[i * h->dstride];
+      std::memcpy(row, q + i * dim, sizeof(float) * dim);
+      if (h->func == kNormCos) {
+        float sum = 0.f;
+        for (int64_t j = 0; j < dim; ++j) sum = sum + row[j] * row[j];
+        if (sum != 0.0f) {
+          const float inv = 1.0f / std::sqrt(sum);
+          for (int64_t j = 0; j < dim; ++j) row[j] = row[j] * inv;
+        }
+      }
+    }
+    const int64_t qblocks = (nq + 31) / 32;
+    int nchunks = (int)std::max<int64_t>(1, std::min<int64_t>(16, (2048 + qblocks - 1) / qblocks));
+    while (nchunks > 1 && (int64_t)h->n / nchunks < 64) --nchunks;
+    h->d_bq.upload(qp.data(), qp.size(), s);
+    h->d_bpd.ensure((size_t)nq * nchunks * 2 * k);
+    h->d_bpi.ensure((size_t)nq * nchunks * 2 * k);
+    h->d_ids.ensure((size_t)nq * k);
+    h->d_dists.ensure((size_t)nq * k);
+    h->d_counts.ensure((size_t)nq);
+    hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
+    hip_check(launch_brute(h->d_vec.p, h->d_xnorm.p, h->d_bq.p, h->n, nq, h->dstride, h->func, k,
+                           nchunks, h->d_bpd.p, h->d_bpi.p, h->d_ext.p, h->d_ids.p, h->d_dists.p,
+                           h->d_counts.p, s),
+              "brute force launch");
+    hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
+    hip_check(hipMemcpyAsync(ids, h->d_ids.p, sizeof(int32_t) * nq * k, hipMemcpyDeviceToHost, s), "D2H");
+    hip_check(hipMemcpyAsync(dists, h->d_dists.p, sizeof(float) * nq * k, hipMemcpyDeviceToHost, s), "D2H");
+    hip_check(hipMemcpyAsync(counts, h->d_counts.p, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, s), "D2H");
+    hip_check(hipStreamSynchronize(s), "brute force");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
+    h->last_ms = ms;
+  });
+}
