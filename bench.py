@@ -324,6 +324,21 @@ def main():
             exact_out = (res["efSearch"], ids_np, dists_np)
     head = max(results.values(), key=lambda r: r["qps"])
 
+    # the dense contraction of the path: exact kNN by brute force on MFMA
+    # (device seq_search), timed by HIP events; FP32 MFMA peak 157.3 TFLOP/s
+    idx.setPrecision("fp32", 0)
+    idx.setBeam(0)
+    bf_ids, bf_d, _ = idx.bruteForce(q, 10)
+    bf_ms = idx.lastStats()["kernel_ms"]
+    bf_flops = 2.0 * len(q) * 1_000_000 * 128
+    ci, cd = _canon(bf_ids, bf_d)
+    bf_exact = bool(np.array_equal(cd, gd[:, :10]) and np.array_equal(ci, gids[:, :10]))
+    brute = {"kernel": "bf_tile_kernel + bf_merge_kernel", "queries": len(q), "k": 10,
+             "ms": round(bf_ms, 3), "tflops": round(bf_flops / bf_ms / 1e9, 1),
+             "frac_of_fp32_mfma_peak": round(bf_flops / bf_ms / 1e9 / 157.3, 4),
+             "equals_ground_truth": bf_exact}
+    log(f"[bench] brute force {brute}")
+
     serve = None
     if args.serve:
         rates = [float(r) for r in args.serve.split(",") if r]
@@ -369,6 +384,7 @@ def main():
                      "alg_bytes_per_launch": head["alg_bytes_per_launch"]},
         "cpu_baseline": cpu,
         "serving": serve,
+        "bruteforce": brute,
     }
     print(json.dumps(out), flush=True)
 

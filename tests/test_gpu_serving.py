@@ -54,3 +54,22 @@ def test_server_rejects_out_of_budget_batches():
     ids, dists, counts = srv.fetch(0)
     assert counts.tolist() == [10, 10]
     srv.close()
+
+
+def test_autotune_picks_a_form_within_the_recall_slack():
+    from nmslib_amd import autotune
+    name = "u128_l2"
+    idx = load(name, F.index_path(name), "l2")
+    q = F.results(name)["queries"]
+    rng = np.random.default_rng(3)
+    ks = rng.choice([1, 10, 100], size=len(q)).astype(np.int32)
+    efs = rng.integers(32, 800, size=len(q)).astype(np.int32)
+    best = autotune.autotune(idx, q, ks, efs, reps=1)
+    exact = next(c for c in best["candidates"] if c["form"] == "fp32-exact")
+    assert best["recall"] >= exact["recall"] - 0.003
+    assert best["qps"] >= max(c["qps"] for c in best["candidates"]
+                              if c["recall"] >= exact["recall"] - 0.003) - 1e-9
+    print({c["form"]: (round(c["qps"]), round(c["recall"], 4)) for c in best["candidates"]})
+    # the choice is applied: a second call hits the cache
+    again = autotune.autotune(idx, q, ks, efs)
+    assert again["form"] == best["form"]
