@@ -156,6 +156,9 @@ FORMS = {
     "fp32-exact": ("fp32", 0, 0),
     "fp32-beam4": ("fp32", 0, 4),
     "bf16-beam4-rerank": ("bf16", 2, 4),
+    "fp32-beam8": ("fp32", 0, 8),
+    "fp32-beam16": ("fp32", 0, 16),
+    "bf16-beam16-rerank": ("bf16", 2, 16),
 }
 
 
@@ -210,6 +213,7 @@ def measure(idx, q, gids, gd, form, args, dev, log_prefix):
     torch.cuda.synchronize()
     st = idx.lastStats()
     lat = idx.lastQueryTimesUs(nq)
+    phases = idx.lastPhaseUs() if beam else None
     idx.collectStats(False)
     ids_np = dids.cpu().numpy()
     dists_np = dd.cpu().numpy()
@@ -249,6 +253,9 @@ def measure(idx, q, gids, gd, form, args, dev, log_prefix):
         "achieved_gbs": round(alg_bytes / avg_kernel_s / 1e9, 1),
         "curve": curve,
     }
+    if phases:
+        tot = sum(phases.values()) or 1.0
+        res["step_phase_share"] = {p: round(v / tot, 3) for p, v in phases.items()}
     log(f"{log_prefix} {form}: {res}")
     return res, ids_np, dists_np
 

@@ -67,6 +67,7 @@ SIGNATURES = {
     "nmsamd_last_query_us": (C.c_int, [_P, _P, _I64]),
     "nmsamd_set_beam": (C.c_int, [_P, C.c_int]),
     "nmsamd_brute_force": (C.c_int, [_P, _P, _I64, _I64, _I32, _P, _P, _P]),
+    "nmsamd_last_phase_us": (C.c_int, [_P, _P]),
 }
 
 
@@ -179,7 +180,8 @@ class FloatIndex:
         _check(load_library().nmsamd_set_precision(self._h, code, int(rerank_mult)))
 
     def setBeam(self, beam: int = 0):
-        """0: exact V1Merge (the reference's results); 1..4: beam mode."""
+        """0: exact V1Merge (the reference's results); 1..16: beam mode
+        (expansions per step, capped at 512 // maxM0)."""
         _check(load_library().nmsamd_set_beam(self._h, int(beam)))
 
     def collectStats(self, on: bool = True):
@@ -193,6 +195,14 @@ class FloatIndex:
         d = {k: int(v) for k, v in zip(keys, out)}
         d["kernel_ms"] = ms.value
         return d
+
+    def lastPhaseUs(self):
+        """Beam mode with collectStats(True): device time per step phase, us
+        summed over the queries of the last search."""
+        out = np.zeros(6, np.float64)
+        _check(load_library().nmsamd_last_phase_us(self._h, _ptr(out)))
+        keys = ("pick", "adjacency_visited", "distances", "filter", "sort_merge", "advance")
+        return {k: float(v) for k, v in zip(keys, out)}
 
     def lastQueryTimesUs(self, nq: int):
         """Per-query device search time (us) of the last search made with

@@ -60,6 +60,9 @@ struct SearchStats {
   unsigned long long dist_upper;
   unsigned long long list_ints_read;
   unsigned long long tie_paths;
+  // beam mode: 100 MHz ticks summed over queries per phase of a step
+  // (pick, adjacency+visited, distances, filter, sort+merge, advance)
+  unsigned long long phase_ticks[6];
 };
 
 struct SearchArgs {
@@ -81,7 +84,9 @@ struct SearchArgs {
   int storage;               // Storage
   int rerank;                // > 0: exact FP32 rerank of the best rerank*k candidates
   int beam;                  // > 0: beam mode, expansions per step (extension)
-  int64_t slot_base;         // first visited-mark slot of this launch
+  int ent;                   // per-step neighbour entries the LDS scratch holds
+                             // (a multiple of 64, at least 128)
+  int64_t slot_base;        // first visited-mark slot of this launch
   const int32_t* order;      // optional: the order in which waves take queries
   const int64_t* nq_dev;     // optional: the query count read on the device
                              // (a replayed graph keeps its kernel arguments)
@@ -94,6 +99,10 @@ struct SearchArgs {
 
 // algoType (hnsw.cc:491-501); hybrid picks SearchOld for ef >= 1000 per query
 enum AlgoMode : int { kAlgoHybrid = 0, kAlgoOld = 1, kAlgoV1Merge = 2 };
+
+// Kernel variants: every search mode, or beam mode only (lower register
+// pressure), the latter also with all of a step's vector loads in flight.
+enum KernelMode : int { kModeAll = 0, kModeBeam = 1, kModeBeamWide = 2 };
 
 }  // namespace nmsamd
 #endif

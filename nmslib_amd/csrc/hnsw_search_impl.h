@@ -420,42 +420,44 @@ __device__ __forceinline__ void dist_cands(const float* __restrict__ qs, const D
   }
 }
 
-constexpr int kScratch = 128;  // per-step buffers: up to 128 new neighbours
-constexpr int kHashSlots = 256;
-
-// Per-wave LDS: the sorted array, the query, and two 1 KB scratch regions
-// whose uses do not overlap in time within a step:
-//   region X: dedupe table -> fresh neighbours + distances -> sorted buffer
-//   region Y: picks of the beam step -> filtered buffer (list order)
+// Per-wave LDS: the sorted array, the query, two scratch regions of 8 * ent
+// bytes (ent = neighbour entries per step, a multiple of 64, >= 128) whose
+// uses do not overlap in time within a step, and the merge's position bitmap:
+//   region X: dedupe table (2 * ent slots) -> fresh neighbours + distances
+//             -> sorted buffer of the exact search
+//   region Y: picks of the beam step -> filtered items -> sorted items
 struct WaveLds {
   uint2* A;        // [cap] sorted array: x = key bits, y = internal id | kUsedBit
   float* qs;       // [dstride] query
-  uint32_t* htab;  // X [kHashSlots] per-step dedupe table of the beam search
-  int* cand;       // X [kScratch] fresh neighbours of this step (list order)
-  float* cdist;    // X [kScratch] their distances
-  float* skey;     // X [kScratch] buffer (sorted)
+  uint32_t* htab;  // X [2 * ent] per-step dedupe table of the beam search
+  int* cand;       // X [ent] fresh neighbours of this step (list order)
+  float* cdist;    // X [ent] their distances
+  float* skey;     // X [ent] buffer (sorted)
   int* sid;        // X
   int* picks;      // Y [beam] nodes expanded by a beam step
-  float* bkey;     // Y [kScratch] buffer (list order)
+  float* bkey;     // Y [ent] buffer (list order)
   int* bid;        // Y
+  uint2* items;    // Y [ent] beam step: admitted items, then the same sorted
+  uint4* meta;     // [cap / 64 + 2] beam merge: per chunk, which positions new
+                   // items take (x, y) and how many land below it (z)
 };
 
-constexpr int kScratchBytes = 2 * kScratch * 8;  // regions X and Y
-
-__device__ __forceinline__ WaveLds carve(unsigned char* base, int cap, int dstride) {
+__device__ __forceinline__ WaveLds carve(unsigned char* base, int cap, int dstride, int ent) {
   WaveLds w;
   w.A = reinterpret_cast<uint2*>(base);
   w.qs = reinterpret_cast<float*>(w.A + ((cap + 1) & ~1));  // keeps 16-byte alignment
   int* x = reinterpret_cast<int*>(w.qs + dstride);
-  int* y = x + 2 * kScratch;
+  int* y = x + 2 * ent;
   w.htab = reinterpret_cast<uint32_t*>(x);
   w.cand = x;
-  w.cdist = reinterpret_cast<float*>(x + kScratch);
+  w.cdist = reinterpret_cast<float*>(x + ent);
   w.skey = reinterpret_cast<float*>(x);
-  w.sid = x + kScratch;
+  w.sid = x + ent;
   w.picks = y;
   w.bkey = reinterpret_cast<float*>(y);
-  w.bid = y + kScratch;
+  w.bid = y + ent;
+  w.items = reinterpret_cast<uint2*>(y);
+  w.meta = reinterpret_cast<uint4*>(y + 2 * ent);
   return w;
 }
 
@@ -908,12 +910,195 @@ __device__ __forceinline__ int lower_count_kid(const uint2* A, int n, float v, u
   return lo;
 }
 
+// The value lane (lane ^ d) holds.  d = 1, 2: DPP quad permutes; 4, 8: DPP
+// row shifts (row_shl:d brings lane i + d, row_shr:d lane i - d, within rows
+// of 16); 16: ds_swizzle in bit mode (and 0x1f, xor 16); 32: a permute
+// through the LDS crossbar.  d is a constant after unrolling.
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v, int d, int lane) {
+  switch (d) {
+    case 1: return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
+    case 2: return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);
+    case 4: {
+      const int up = __builtin_amdgcn_update_dpp(0, (int)v, 0x104, 0xF, 0xF, false);
+      const int dn = __builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);
+      return (uint32_t)((lane & 4) ? dn : up);
+    }
+    case 8: {
+      const int up = __builtin_amdgcn_update_dpp(0, (int)v, 0x108, 0xF, 0xF, false);
+      const int dn = __builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);
+      return (uint32_t)((lane & 8) ? dn : up);
+    }
+    case 16: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (16 << 10));
+    default: return (uint32_t)__shfl_xor((int)v, d);
+  }
+}
+
+// Sorts the 64 * R items held as item r * 64 + lane = (k[r], id[r]) into
+// ascending (key, id) order with Batcher's bitonic network: partners 64 or
+// more apart sit in the same lane, nearer ones are exchanged across lanes.
+template <int R>
+__device__ __forceinline__ void bitonic_sort(float (&k)[R], uint32_t (&id)[R], int lane) {
+#pragma unroll
+  for (int s = 2; s <= 64 * R; s <<= 1) {
+#pragma unroll
+    for (int d = s >> 1; d > 0; d >>= 1) {
+      if (d >= 64) {
+        const int dr = d >> 6;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if ((r & dr) == 0) {
+            const int r2 = r | dr;
+            const bool asc = ((r << 6) & s) == 0;
+            const bool sw = asc ? kid_less(k[r2], id[r2], k[r], id[r])
+                                : kid_less(k[r], id[r], k[r2], id[r2]);
+            if (sw) {
+              const float tk = k[r];
+              k[r] = k[r2];
+              k[r2] = tk;
+              const uint32_t ti = id[r];
+              id[r] = id[r2];
+              id[r2] = ti;
+            }
+          }
+        }
+      } else {
+        const bool lower = (lane & d) == 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const bool asc = (((r << 6) | lane) & s) == 0;
+          const float pk = __uint_as_float(xor_lane(__float_as_uint(k[r]), d, lane));
+          const uint32_t pi = xor_lane(id[r], d, lane);
+          const bool pl = kid_less(pk, pi, k[r], id[r]);
+          // the lower slot keeps the smaller item when ascending
+          if (lower == asc ? pl : !pl) {
+            k[r] = pk;
+            id[r] = pi;
+          }
+        }
+      }
+    }
+  }
+}
+
+// Sorts the m (<= 64 R) items of X[0..m) by (key, id), writes them back in
+// order and merges them into A[0..n), truncating at C.  Returns the position
+// the smallest new item takes (>= C: none is kept).
+//
+// New item j lands at P_j = j + #{A items before it} (a binary search per
+// item, all rows in step).  Per output chunk c of 64 positions, from the chunk
+// of P_0 up, meta[c - clo] holds the bitmap of the new items' positions in
+// the chunk (x, y) and the number of new items below it (z).  Output position
+// p then takes new item #{new items below p} if marked, else A[p - #{new items
+// below p}].  Output chunk c reads only below 64 (c + 1) and writes
+// [64 c, 64 (c + 1)), so going top-down the reads of lower chunks may run ahead
+// of the writes: eight chunks go in flight at once, with no lane exchange.
+template <int R>
+__device__ __forceinline__ int sort_merge(uint2* A, int n_, int C_, uint2* X, int m_,
+                                          uint4* meta, int lane) {
+  const int n = __builtin_amdgcn_readfirstlane(n_);
+  const int C = __builtin_amdgcn_readfirstlane(C_);
+  const int m = __builtin_amdgcn_readfirstlane(m_);
+  float k[R];
+  uint32_t id[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int j = r * 64 + lane;
+    if (j < m) {
+      const uint2 v = X[j];
+      k[r] = __uint_as_float(v.x);
+      id[r] = v.y;
+    } else {
+      k[r] = kInf();
+      id[r] = 0xffffffffu;
+    }
+  }
+  bitonic_sort<R>(k, id, lane);
+  WSYNC();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int j = r * 64 + lane;
+    if (j < m) X[j] = make_uint2(__float_as_uint(k[r]), id[r]);
+  }
+  int L[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) L[r] = 0;
+  for (int step = 1 << (31 - __builtin_clz((unsigned)n)); step > 0; step >>= 1) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int i = L[r] + step - 1;
+      if (r * 64 + lane < m && i < n) {
+        const uint2 e = A[i];
+        if (kid_less(__uint_as_float(e.x), e.y & ~kUsedBit, k[r], id[r])) L[r] += step;
+      }
+    }
+  }
+  const int nout = min(C, n + m);
+  const int P0 = __builtin_amdgcn_readlane(L[0], 0);
+  if (P0 >= nout) return P0;
+  const int clo = P0 >> 6, chi = (nout - 1) >> 6;
+  for (int c = lane; c <= chi - clo; c += 64) meta[c] = make_uint4(0u, 0u, 0u, 0u);
+  WSYNC();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int j = r * 64 + lane;
+    const int p = L[r] + j;
+    if (j < m && p < nout)
+      atomicOr(reinterpret_cast<unsigned long long*>(&meta[(p >> 6) - clo]), 1ull << (p & 63));
+  }
+  WSYNC();
+  const uint64_t below_lane = (1ull << lane) - 1;
+  // new items below each chunk: an exclusive prefix over the chunks' counts,
+  // 64 chunks at a time, bit-sliced through ballots
+  int base = 0;
+  for (int g = 0; g <= chi - clo; g += 64) {
+    const int c = g + lane;
+    const bool in = c <= chi - clo;
+    const uint4 e = in ? meta[c] : make_uint4(0u, 0u, 0u, 0u);
+    const int cnt = __popc(e.x) + __popc(e.y);
+    int excl = base, tot = 0;
+#pragma unroll
+    for (int b = 0; b < 7; ++b) {
+      const uint64_t mb = __ballot((cnt >> b) & 1);
+      excl += __popcll(mb & below_lane) << b;
+      tot += __popcll(mb) << b;
+    }
+    if (in) meta[c].z = (uint32_t)excl;
+    base += tot;
+  }
+  WSYNC();
+  for (int c = chi; c >= clo; c -= 8) {
+    uint4 md[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) md[u] = meta[max(c - u, clo) - clo];
+    uint2 v[8];
+    int pos[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int cc = c - u;
+      const int p = (cc << 6) + lane;
+      const uint64_t w = ((uint64_t)md[u].y << 32) | md[u].x;
+      const int bl = (int)md[u].z + __popcll(w & below_lane);
+      const bool ok = cc >= clo && p < nout;
+      const uint2* src = ((w >> lane) & 1ull) ? X + bl : A + (p - bl);
+      v[u] = *(ok ? src : A);
+      pos[u] = ok ? p : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (pos[u] >= 0) A[pos[u]] = v[u];
+  }
+  return P0;
+}
+
 template <int D, int FUNC, int ST, int WIDE>
 __device__ void beam_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, uint8_t ep,
                            int qi, int k, int ef, int lane, int h, uint64_t ltm) {
   const DevIndex& ix = a.ix;
   const int C = max(ef, k);
-  const int W = max(1, min(a.beam, kScratch / ix.maxM0));
+  const int E = a.ent;  // entries per step: W * maxM0 <= E <= 512
+  const int W = max(1, min(a.beam, E / ix.maxM0));
+  const int erows = E >> 6;
+  const uint32_t hslots = 2u * (uint32_t)E;
   uint2* A = w.A;
   unsigned long long nexp = 0, nd0 = 0, ndu = 1, nl = 0;
 
@@ -928,6 +1113,17 @@ __device__ void beam_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, 
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   int ce = 0;
+  // per-phase device time (100 MHz ticks) when counters are collected
+  const bool timed = a.stats != nullptr;
+  unsigned long long ph[6] = {0, 0, 0, 0, 0, 0};
+  uint64_t tmark = timed ? __builtin_amdgcn_s_memrealtime() : 0;
+  auto lap = [&](int p) {
+    if (timed) {
+      const uint64_t now = __builtin_amdgcn_s_memrealtime();
+      ph[p] += now - tmark;
+      tmark = now;
+    }
+  };
   while (ce < min(n, ef)) {
     WSYNC();
     // the first W unexpanded items below min(n, ef), in position order
@@ -949,114 +1145,104 @@ __device__ void beam_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, 
     nexp += nsel;
     const float topk = akey(A, n - 1);
     const int nsnap = n;
-    // clear the dedupe table while the picks land
-    reinterpret_cast<uint4*>(w.htab)[lane] = make_uint4(~0u, ~0u, ~0u, ~0u);
+    // clear the dedupe table (2 E slots) while the picks land
+    for (int i = lane; i < E; i += 64) reinterpret_cast<uint2*>(w.htab)[i] = make_uint2(~0u, ~0u);
     WSYNC();
+    lap(0);
 
-    // adjacency rows of the picks, two entries per lane (W * maxM0 <= 128)
-    int t[2];
-    bool fresh[2];
+    // adjacency rows of the picks, entry e = 64 r + lane (W * maxM0 <= E)
+    int t[8];
+    bool fresh[8];
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const int e = lane + 64 * r;
-      const int li = e / ix.maxM0;
-      const int sl = e - li * ix.maxM0;
-      t[r] = li < nsel ? ix.l0[(int64_t)w.picks[li] * ix.l0stride + sl] : -1;
+    for (int r = 0; r < 8; ++r) {
+      t[r] = -1;
+      if (r < erows) {
+        const int e = lane + 64 * r;
+        const int li = e / ix.maxM0;
+        const int sl = e - li * ix.maxM0;
+        if (li < nsel) t[r] = ix.l0[(int64_t)w.picks[li] * ix.l0stride + sl];
+      }
     }
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const bool valid = t[r] >= 0;
-      nl += __popcll(__ballot(valid));
-      fresh[r] = valid && vis[t[r]] != ep;
+    for (int r = 0; r < 8; ++r) {
+      fresh[r] = false;
+      if (r < erows) {
+        const bool valid = t[r] >= 0;
+        nl += __popcll(__ballot(valid));
+        fresh[r] = valid && vis[t[r]] != ep;
+      }
     }
     nl += nsel;
     // a neighbour shared by two picks is taken once (LDS hash, linear probing)
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      if (fresh[r]) {
-        uint32_t slot = ((uint32_t)t[r] * 2654435761u) >> 24;
-        for (;;) {
-          const uint32_t old = atomicCAS(&w.htab[slot], ~0u, (uint32_t)t[r]);
-          if (old == ~0u) break;
-          if (old == (uint32_t)t[r]) {
-            fresh[r] = false;
-            break;
+    for (int r = 0; r < 8; ++r) {
+      if (r < erows) {
+        if (fresh[r]) {
+          uint32_t slot =
+              (uint32_t)(((uint64_t)((uint32_t)t[r] * 2654435761u) * hslots) >> 32);
+          for (;;) {
+            const uint32_t old = atomicCAS(&w.htab[slot], ~0u, (uint32_t)t[r]);
+            if (old == ~0u) break;
+            if (old == (uint32_t)t[r]) {
+              fresh[r] = false;
+              break;
+            }
+            if (++slot == hslots) slot = 0;
           }
-          slot = (slot + 1) & (kHashSlots - 1);
         }
+        if (fresh[r]) vis[t[r]] = ep;
       }
-      if (fresh[r]) vis[t[r]] = ep;
     }
-    const uint64_t f0 = __ballot(fresh[0]);
-    const uint64_t f1 = __ballot(fresh[1]);
-    const int c0 = __popcll(f0);
-    const int ncand = c0 + __popcll(f1);
-    if (fresh[0]) w.cand[__popcll(f0 & ltm)] = t[0];
-    if (fresh[1]) w.cand[c0 + __popcll(f1 & ltm)] = t[1];
+    int ncand = 0;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      if (r < erows) {
+        const uint64_t fm = __ballot(fresh[r]);
+        if (fresh[r]) w.cand[ncand + __popcll(fm & ltm)] = t[r];
+        ncand += __popcll(fm);
+      }
+    }
     nd0 += ncand;
     WSYNC();
+    lap(1);
     dist_cands<D, FUNC, ST, WIDE>(w.qs, ix, w.cand, w.cdist, ncand, lane, h, cur);
     WSYNC();
+    lap(2);
     // keep what the snapshot admits; order does not matter (sorted next)
     int m = 0;
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const int s = lane + 64 * r;
+    for (int s0 = 0; s0 < ncand; s0 += 64) {
+      const int s = s0 + lane;
       const float dj = s < ncand ? w.cdist[s] : 0.f;
       const bool keep = s < ncand && (dj < topk || nsnap < ef);
       const uint64_t km = __ballot(keep);
-      if (keep) {
-        const int o = m + __popcll(km & ltm);
-        w.bkey[o] = dj;
-        w.bid[o] = w.cand[s];
-      }
+      if (keep) w.items[m + __popcll(km & ltm)] = make_uint2(__float_as_uint(dj), (uint32_t)w.cand[s]);
       m += __popcll(km);
     }
     WSYNC();
+    lap(3);
     if (m > 0) {
-      // rank sort by (distance, id); keys broadcast by readlane
-      const float k0 = lane < m ? w.bkey[lane] : 0.f;
-      const uint32_t i0 = lane < m ? (uint32_t)w.bid[lane] : 0u;
-      const float k1 = lane + 64 < m ? w.bkey[lane + 64] : 0.f;
-      const uint32_t i1 = lane + 64 < m ? (uint32_t)w.bid[lane + 64] : 0u;
-      int r0 = 0, r1 = 0;
-      for (int j = 0; j < m; ++j) {
-        const bool lo = j < 64;
-        const float kj = __int_as_float(
-            __builtin_amdgcn_readlane(__float_as_int(lo ? k0 : k1), j & 63));
-        const uint32_t ij = (uint32_t)__builtin_amdgcn_readlane((int)(lo ? i0 : i1), j & 63);
-        r0 += kid_less(kj, ij, k0, i0);
-        r1 += kid_less(kj, ij, k1, i1);
-      }
-      if (lane < m) {
-        w.skey[r0] = k0;
-        w.sid[r0] = (int)i0;
-      }
-      if (lane + 64 < m) {
-        w.skey[r1] = k1;
-        w.sid[r1] = (int)i1;
-      }
-      WSYNC();
-      // merge in batches of 64 (later batches sort after earlier ones)
-      for (int b = 0; b < m; b += 64) {
-        const int j = b + lane;
-        const bool in = j < m;
-        const float bj = in ? w.skey[j] : 0.f;
-        const uint32_t ij = in ? (uint32_t)w.sid[j] : 0u;
-        const int Lj = in ? lower_count_kid(A, n, bj, ij) : 0x7fffffff;
-        const int cnt = min(64, m - b);
-        const int L0 = __shfl(Lj, 0);
-        place_items(A, n, C, lane, cnt, Lj, make_uint2(__float_as_uint(bj), ij));
+      // sort by (distance, id) and merge, at most 256 items per pass (a later
+      // pass only holds items the earlier ones do not)
+      for (int b = 0; b < m; b += 256) {
+        const int cnt = min(256, m - b);
+        int L0;
+        if (cnt <= 64) L0 = sort_merge<1>(A, n, C, w.items + b, cnt, w.meta, lane);
+        else if (cnt <= 128) L0 = sort_merge<2>(A, n, C, w.items + b, cnt, w.meta, lane);
+        else L0 = sort_merge<4>(A, n, C, w.items + b, cnt, w.meta, lane);
         n = min(C, n + cnt);
         if (L0 < ce) ce = L0;
         WSYNC();
       }
+      lap(4);
     }
     WSYNC();
     ce = first_unused(A, ce, n, lane);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    lap(5);
   }
   WSYNC();
+  if (timed && lane == 0)
+    for (int p = 0; p < 6; ++p) atomicAdd(&a.stats->phase_ticks[p], ph[p]);
   const QueryCounters c{nexp, nd0, ndu, nl, 0};
   finish_query<D, FUNC, ST>(a, w, n, qi, k, lane, h, c);
 }
@@ -1208,10 +1394,13 @@ __device__ void old_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, u
   }
 }
 
-// WIDE = 1: the beam mode keeps a round's vector loads in flight together
-// (~240 VGPRs, one wave per SIMD) -- used when the LDS footprint already
-// limits residency to about a wave per SIMD; WIDE = 0 keeps occupancy high.
-template <int D, int FUNC, int ST, int WIDE>
+// MODE = kModeAll runs every search mode.  kModeBeam / kModeBeamWide hold
+// only the beam search (launched when no query of the batch takes SearchOld),
+// so its loop does not carry the exact search's register state;
+// kModeBeamWide keeps a round's vector loads in flight together (~240 VGPRs,
+// one wave per SIMD) and is used when the LDS footprint already limits
+// residency to about a wave per SIMD.
+template <int D, int FUNC, int ST, int MODE>
 __global__ void __launch_bounds__(kWave* kWavesPerBlock)
     hnsw_search_kernel(SearchArgs a, int lds_wave_bytes) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -1219,7 +1408,7 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock)
   const int lane = threadIdx.x & 63;
   const int h = lane & 1;
   const uint64_t ltm = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  const WaveLds w = carve(smem + wave * lds_wave_bytes, a.cap, a.ix.dstride);
+  const WaveLds w = carve(smem + wave * lds_wave_bytes, a.cap, a.ix.dstride, a.ent);
   const int64_t slot = a.slot_base + (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
   uint8_t* vis = a.ix.visited + slot * a.ix.vstride;
   const int dim = a.ix.dim;
@@ -1275,13 +1464,17 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock)
     }
     // Hnsw::Search dispatch (hnsw.cc:724): SearchOld for algoType=old, or
     // hybrid with ef >= 1000
-    const bool use_old = a.algo == kAlgoOld || (a.algo == kAlgoHybrid && ef >= 1000);
-    if (use_old)
-      old_query<D, FUNC, ST>(a, w, vis, ep, qi, k, ef, lane, h, ltm, slot);
-    else if (a.beam > 0)
-      beam_query<D, FUNC, ST, WIDE>(a, w, vis, ep, qi, k, ef, lane, h, ltm);
-    else
-      v1merge_query<D, FUNC, ST>(a, w, vis, ep, qi, k, ef, lane, h, ltm);
+    if constexpr (MODE == kModeAll) {
+      const bool use_old = a.algo == kAlgoOld || (a.algo == kAlgoHybrid && ef >= 1000);
+      if (use_old)
+        old_query<D, FUNC, ST>(a, w, vis, ep, qi, k, ef, lane, h, ltm, slot);
+      else if (a.beam > 0)
+        beam_query<D, FUNC, ST, 0>(a, w, vis, ep, qi, k, ef, lane, h, ltm);
+      else
+        v1merge_query<D, FUNC, ST>(a, w, vis, ep, qi, k, ef, lane, h, ltm);
+    } else {
+      beam_query<D, FUNC, ST, MODE == kModeBeamWide>(a, w, vis, ep, qi, k, ef, lane, h, ltm);
+    }
     WSYNC();
     if (a.q_ticks && lane == 0)
       a.q_ticks[qi] = (uint32_t)(__builtin_amdgcn_s_memrealtime() - t_start);

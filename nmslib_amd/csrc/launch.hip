@@ -9,16 +9,16 @@
 namespace nmsamd {
 
 using KernelFn = void (*)(SearchArgs, int);
-KernelFn pick_kernel_f32(int dim, int func, int wide);
-KernelFn pick_kernel_bf16(int dim, int func, int wide);
-KernelFn pick_kernel_fp8(int dim, int func, int wide);
+KernelFn pick_kernel_f32(int dim, int func, int mode);
+KernelFn pick_kernel_bf16(int dim, int func, int mode);
+KernelFn pick_kernel_fp8(int dim, int func, int mode);
 
 hipError_t launch_search(const SearchArgs& a, int blocks, int waves_per_block, int lds_wave_bytes,
-                         int wide, hipStream_t stream) {
+                         int mode, hipStream_t stream) {
   KernelFn fn = nullptr;
-  if (a.storage == kF32) fn = pick_kernel_f32(a.ix.dim, a.ix.func, wide);
-  else if (a.storage == kBF16) fn = pick_kernel_bf16(a.ix.dim, a.ix.func, wide);
-  else fn = pick_kernel_fp8(a.ix.dim, a.ix.func, wide);
+  if (a.storage == kF32) fn = pick_kernel_f32(a.ix.dim, a.ix.func, mode);
+  else if (a.storage == kBF16) fn = pick_kernel_bf16(a.ix.dim, a.ix.func, mode);
+  else fn = pick_kernel_fp8(a.ix.dim, a.ix.func, mode);
   if (!fn) return hipErrorInvalidValue;
   if (waves_per_block < 1 || waves_per_block > kWavesPerBlock) return hipErrorInvalidValue;
   const size_t lds = (size_t)lds_wave_bytes * waves_per_block;

@@ -27,7 +27,7 @@
 
 namespace nmsamd {
 hipError_t launch_search(const SearchArgs& a, int blocks, int waves_per_block, int lds_wave_bytes,
-                         int wide, hipStream_t stream);
+                         int mode, hipStream_t stream);
 hipError_t launch_fill_i32(int* p, int v, int n, hipStream_t stream);
 hipError_t launch_convert_lowp(const float* vec, uint16_t* vec16, uint8_t* vec8, float* scale8,
                                int64_t n, int dim, int dstride, int storage, hipStream_t stream);
@@ -351,8 +351,20 @@ DevIndex dev_index(nmsamd_index* h) {
   return ix;
 }
 
-int lds_wave_bytes(int cap, int dstride) {
-  const int b = ((cap + 1) & ~1) * 8 + dstride * 4 + 2048;  // + two 1 KB scratch regions
+// Neighbour entries one search step holds in LDS: beam mode takes up to
+// kMaxStepEntries / maxM0 expansions per step, W * maxM0 entries rounded up to
+// whole rows of 64; the other modes expand one list of <= 64 entries.
+constexpr int kMaxStepEntries = 512;
+int step_entries(const nmsamd_index* h) {
+  if (h->beam <= 0) return 128;
+  const int w = std::max(1, std::min(h->beam, kMaxStepEntries / (int)h->maxM0));
+  return std::max(128, (w * (int)h->maxM0 + 63) / 64 * 64);
+}
+
+// sorted array + query + two scratch regions of 8 * ent bytes + the merge's
+// 16-byte record per 64 positions
+int lds_wave_bytes(int cap, int dstride, int ent) {
+  const int b = ((cap + 1) & ~1) * 8 + dstride * 4 + 16 * ent + (cap / 64 + 2) * 16;
   return (b + 15) / 16 * 16;
 }
 
@@ -371,7 +383,8 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
     const int r = std::min(std::max(max_ef, max_k), std::max(max_k, h->rerank_mult * max_k));
     cap = std::max(cap, std::max(max_ef, max_k) + r);
   }
-  const int lds = lds_wave_bytes(cap, h->dstride);
+  const int ent = step_entries(h);
+  const int lds = lds_wave_bytes(cap, h->dstride, ent);
   if ((size_t)lds > kLdsPerCu)
     fail(NMSAMD_EINVAL, "efSearch / k too large: the search array of " + std::to_string(cap) +
                             " items exceeds the LDS of one wave");
@@ -426,15 +439,18 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   a.storage = h->storage;
   a.rerank = rerank ? h->rerank_mult : 0;
   a.beam = h->beam;
+  a.ent = ent;
   a.algo = h->algo == kOld ? kAlgoOld : (h->algo == kV1Merge ? kAlgoV1Merge : kAlgoHybrid);
   a.old_heap_key = h->d_oldk.p;
   a.old_heap_id = h->d_oldi.p;
   a.old_heap_cap = heap_cap;
   hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
-  // beam mode keeps a round's gathers in flight together when the LDS
-  // footprint already holds residency to about one wave per SIMD
-  const int wide = (h->beam > 0 && wpb * per_cu <= 8) ? 1 : 0;
-  hip_check(launch_search(a, blocks, wpb, lds, wide, s), "search kernel launch");
+  // a beam batch with no SearchOld query takes the beam-only kernel, which
+  // keeps a round's gathers in flight together when the LDS footprint
+  // already holds residency to about one wave per SIMD
+  const int mode = (h->beam > 0 && !any_old)
+                       ? (wpb * per_cu <= 8 ? kModeBeamWide : kModeBeam) : kModeAll;
+  hip_check(launch_search(a, blocks, wpb, lds, mode, s), "search kernel launch");
   hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
   h->pending = true;
 }
@@ -736,7 +752,7 @@ int nmsamd_set_precision(nmsamd_index* h, int storage, int rerank_mult) {
 int nmsamd_set_beam(nmsamd_index* h, int beam) {
   return guarded([&] {
     if (!h) fail(NMSAMD_EINVAL, "null handle");
-    if (beam < 0 || beam > 4) fail(NMSAMD_EINVAL, "beam must be 0 (exact V1Merge) or 1..4");
+    if (beam < 0 || beam > 16) fail(NMSAMD_EINVAL, "beam must be 0 (exact V1Merge) or 1..16");
     h->beam = beam;
   });
 }
@@ -768,6 +784,13 @@ int nmsamd_last_query_us(const nmsamd_index* h, float* out_us, int64_t nq) {
     hip_check(hipMemcpy(t.data(), h->d_qticks.p, sizeof(uint32_t) * nq, hipMemcpyDeviceToHost),
               "query times");
     for (int64_t i = 0; i < nq; ++i) out_us[i] = (float)(t[i] * 0.01);  // 100 MHz clock
+  });
+}
+
+int nmsamd_last_phase_us(const nmsamd_index* h, double* out6) {
+  return guarded([&] {
+    if (!h || !out6) fail(NMSAMD_EINVAL, "null argument");
+    for (int p = 0; p < 6; ++p) out6[p] = h->last_stats.phase_ticks[p] * 0.01;  // 100 MHz
   });
 }
 
@@ -815,7 +838,7 @@ struct ServerLane {
 
 struct nmsamd_server {
   nmsamd_index* h = nullptr;
-  int max_queries = 0, max_ef = 0, max_k = 0, cap = 0, lds = 0, wpb = 0, wide = 0;
+  int max_queries = 0, max_ef = 0, max_k = 0, cap = 0, ent = 0, lds = 0, wpb = 0, wide = 0;
   int slots_per_lane = 0;
   bool use_graph = false;
   DevBuf<uint8_t> visited, epoch;
@@ -858,6 +881,7 @@ SearchArgs server_args(nmsamd_server* s, ServerLane& L) {
   a.storage = h->storage;
   a.rerank = (h->storage != kF32 && h->rerank_mult > 0) ? h->rerank_mult : 0;
   a.beam = h->beam;
+  a.ent = s->ent;
   a.algo = kAlgoV1Merge;
   a.slot_base = L.slot_base;
   return a;
@@ -866,7 +890,9 @@ SearchArgs server_args(nmsamd_server* s, ServerLane& L) {
 void lane_launch(nmsamd_server* s, ServerLane& L) {
   SearchArgs a = server_args(s, L);
   hip_check(launch_fill_i32(L.work.p, 0, 2, L.stream), "fill");
-  hip_check(launch_search(a, L.blocks, s->wpb, s->lds, s->wide, L.stream), "search kernel launch");
+  // the server runs V1Merge, so beam batches always take the beam-only kernel
+  const int mode = a.beam > 0 ? (s->wide ? kModeBeamWide : kModeBeam) : kModeAll;
+  hip_check(launch_search(a, L.blocks, s->wpb, s->lds, mode, L.stream), "search kernel launch");
 }
 
 }  // namespace
@@ -891,7 +917,8 @@ int nmsamd_server_open(nmsamd_index* h, int lanes, int max_queries, int max_ef, 
     if (h->storage != kF32 && h->rerank_mult > 0)
       s->cap = std::max(s->cap, std::max(max_ef, max_k) +
                                     std::min(std::max(max_ef, max_k), h->rerank_mult * max_k));
-    s->lds = lds_wave_bytes(s->cap, h->dstride);
+    s->ent = step_entries(h);
+    s->lds = lds_wave_bytes(s->cap, h->dstride, s->ent);
     if ((size_t)s->lds > kLdsPerCu) fail(NMSAMD_EINVAL, "max_ef / max_k exceed the LDS of one wave");
     ensure_lowp(h);
     s->wpb = (int)std::max<size_t>(1, std::min<size_t>(kWavesPerBlock, kLdsPerCu / s->lds));

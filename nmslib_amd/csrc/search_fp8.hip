@@ -5,7 +5,7 @@ namespace nmsamd {
 
 using KernelFn = void (*)(SearchArgs, int);
 
-KernelFn pick_kernel_fp8(int dim, int func, int wide) {
+KernelFn pick_kernel_fp8(int dim, int func, int mode) {
 #define NMS_CASE(D, W)                                                 \
   switch (func) {                                                      \
     case kL2Sqr16: return hnsw_search_kernel<D, kL2Sqr16, kFP8, W>;      \
@@ -21,10 +21,12 @@ KernelFn pick_kernel_fp8(int dim, int func, int wide) {
     case 128: NMS_CASE(128, W)     \
     default: NMS_CASE(0, W)        \
   }
-  if (wide) {
-    NMS_DIMS(1)
+  if (mode == kModeBeamWide) {
+    NMS_DIMS(kModeBeamWide)
+  } else if (mode == kModeBeam) {
+    NMS_DIMS(kModeBeam)
   } else {
-    NMS_DIMS(0)
+    NMS_DIMS(kModeAll)
   }
 #undef NMS_DIMS
 #undef NMS_CASE

@@ -254,16 +254,13 @@ struct OldNode {  // EvaluatedMSWNodeInt (hnsw.h:410-431): ordered by distance o
   bool operator<(const OldNode& o) const { return d < o.d; }
 };
 
-void search_one(const Index& ix, float* q, size_t k, size_t ef, int algo, Visited& vis,
-                ResultHeap& res, ora_stats& st) {
-  if (ix.n == 0) return;
+// Greedy descent through the upper levels (hnsw_distfunc_opt.cc:168-198):
+// rescan the current node's list at each level, moving to any strictly closer
+// neighbour mid-scan, until a full scan changes nothing.
+void descend(const Index& ix, const float* q, int32_t& cur, float& curdist, ora_stats& st) {
   const int64_t d = ix.dim;
-  if (ix.func == ORA_NORMCOS) normalize(q, d);
-  vis.next();
-  const uint8_t V = vis.cur;
-
-  int32_t cur = (int32_t)ix.enter;
-  float curdist = dist_fn(ix.func, q, ix.vec(cur), d);
+  cur = (int32_t)ix.enter;
+  curdist = dist_fn(ix.func, q, ix.vec(cur), d);
   st.dist_upper++;
   for (int lev = ix.maxlevel; lev > 0; --lev) {
     bool changed = true;
@@ -284,6 +281,19 @@ void search_one(const Index& ix, float* q, size_t k, size_t ef, int algo, Visite
       }
     }
   }
+}
+
+void search_one(const Index& ix, float* q, size_t k, size_t ef, int algo, Visited& vis,
+                ResultHeap& res, ora_stats& st) {
+  if (ix.n == 0) return;
+  const int64_t d = ix.dim;
+  if (ix.func == ORA_NORMCOS) normalize(q, d);
+  vis.next();
+  const uint8_t V = vis.cur;
+
+  int32_t cur = 0;
+  float curdist = 0.f;
+  descend(ix, q, cur, curdist, st);
 
   const bool use_old = algo == ORA_ALGO_OLD || (algo == ORA_ALGO_HYBRID && ef >= 1000);
   if (use_old) {
@@ -362,6 +372,76 @@ void search_one(const Index& ix, float* q, size_t k, size_t ef, int algo, Visite
     while (curr < arr.n && arr.v[curr].used) ++curr;
   }
   for (size_t i = 0; i < k && i < arr.n; ++i) res.offer(arr.v[i].key, arr.v[i].data);
+}
+
+// Beam mode of the product (an extension with no reference counterpart; this
+// model is its checker).  V1Merge's array (capacity max(ef, k), stop once the
+// first min(size, ef) items are all expanded) ordered by (distance, internal
+// id); each step expands the first `beam` unexpanded items together: every
+// neighbour unvisited before the step is taken once, the snapshot rule of
+// hnsw_distfunc_opt.cc:240-246 (d < last key, or fewer than ef items) admits
+// it, and the admitted items join the array in (distance, id) order.
+struct BeamItem {
+  float key;
+  int32_t id;
+  bool used;
+  bool operator<(const BeamItem& o) const { return key < o.key || (key == o.key && id < o.id); }
+};
+
+void search_beam_one(const Index& ix, float* q, size_t k, size_t ef, int beam, Visited& vis,
+                     std::vector<BeamItem>& A, ora_stats& st) {
+  A.clear();
+  if (ix.n == 0 || k == 0) return;
+  const int64_t d = ix.dim;
+  if (ix.func == ORA_NORMCOS) normalize(q, d);
+  vis.next();
+  const uint8_t V = vis.cur;
+  int32_t cur = 0;
+  float curdist = 0.f;
+  descend(ix, q, cur, curdist, st);
+  const size_t C = std::max(ef, k);
+  A.push_back({curdist, cur, false});
+  vis.mass[cur] = V;
+  size_t ce = 0;
+  std::vector<int32_t> picks, fresh;
+  std::vector<BeamItem> add;
+  while (ce < std::min(A.size(), ef)) {
+    const size_t lim = std::min(A.size(), ef);
+    picks.clear();
+    for (size_t p = ce; p < lim && (int)picks.size() < beam; ++p)
+      if (!A[p].used) {
+        A[p].used = true;
+        picks.push_back(A[p].id);
+      }
+    st.expansions += (int64_t)picks.size();
+    const float topk = A.back().key;
+    const size_t nsnap = A.size();
+    fresh.clear();
+    for (int32_t e : picks) {
+      const int32_t* lst = ix.l0(e);
+      const int32_t cnt = lst[0];
+      st.list_ints_read += 1 + cnt;
+      for (int32_t j = 1; j <= cnt; ++j) {
+        const int32_t t = lst[j];
+        if (vis.mass[t] != V && std::find(fresh.begin(), fresh.end(), t) == fresh.end())
+          fresh.push_back(t);
+      }
+    }
+    add.clear();
+    for (int32_t t : fresh) {
+      vis.mass[t] = V;
+      const float dd = dist_fn(ix.func, q, ix.vec(t), d);
+      st.dist_level0++;
+      if (dd < topk || nsnap < ef) add.push_back({dd, t, false});
+    }
+    std::sort(add.begin(), add.end());
+    std::vector<BeamItem> merged(A.size() + add.size());
+    std::merge(A.begin(), A.end(), add.begin(), add.end(), merged.begin());
+    if (merged.size() > C) merged.resize(C);
+    A.swap(merged);
+    ce = 0;
+    while (ce < A.size() && A[ce].used) ++ce;
+  }
 }
 
 template <class F>
@@ -449,6 +529,42 @@ int ora_search(void* h, const float* q, int64_t nq, int k, int ef, int algo, int
         ids[i * k + sz] = ix.ext_id(res.q.top().second);
         dists[i * k + sz] = res.q.top().first;
         res.q.pop();
+      }
+    });
+    if (stats) {
+      for (auto& s : st) {
+        stats->expansions += s.expansions;
+        stats->dist_level0 += s.dist_level0;
+        stats->dist_upper += s.dist_upper;
+        stats->list_ints_read += s.list_ints_read;
+      }
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+int ora_search_beam(void* h, const float* q, int64_t nq, int k, int ef, int beam, int num_threads,
+                    int32_t* ids, float* dists, int32_t* counts, ora_stats* stats) {
+  try {
+    const Index& ix = *static_cast<Index*>(h);
+    if (k < 0 || ef < 0 || beam < 1) throw std::invalid_argument("k, ef >= 0 and beam >= 1 required");
+    int threads = num_threads <= 0 ? (int)std::max(1u, std::thread::hardware_concurrency())
+                                   : num_threads;
+    std::vector<Visited> vis;
+    for (int t = 0; t < threads; ++t) vis.emplace_back(ix.n);
+    std::vector<ora_stats> st(threads, ora_stats{0, 0, 0, 0});
+    parallel_for(nq, threads, [&](int64_t i, int t) {
+      std::vector<float> qv(q + i * ix.dim, q + (i + 1) * ix.dim);
+      std::vector<BeamItem> A;
+      search_beam_one(ix, qv.data(), (size_t)k, (size_t)ef, beam, vis[t], A, st[t]);
+      const size_t m = std::min(A.size(), (size_t)k);
+      counts[i] = (int32_t)m;
+      for (int j = 0; j < k; ++j) {
+        ids[i * k + j] = j < (int)m ? ix.ext_id(A[j].id) : -1;
+        dists[i * k + j] = j < (int)m ? A[j].key : 0.f;
       }
     });
     if (stats) {

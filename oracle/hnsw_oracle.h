@@ -56,6 +56,12 @@ const int32_t* ora_upper_list(void* h, int64_t i, int level);
 int ora_search(void* h, const float* q, int64_t nq, int k, int ef, int algo, int num_threads,
                int32_t* ids, float* dists, int32_t* counts, ora_stats* stats);
 
+/* Model of the product's beam mode (no reference counterpart): FP32 vectors,
+ * `beam` expansions per step, array ordered by (distance, internal id).
+ * Same output layout as ora_search. */
+int ora_search_beam(void* h, const float* q, int64_t nq, int k, int ef, int beam, int num_threads,
+                    int32_t* ids, float* dists, int32_t* counts, ora_stats* stats);
+
 float ora_dist(int func, const float* q, const float* x, int64_t dim);
 void ora_normalize(float* v, int64_t dim);
 

@@ -52,6 +52,8 @@ def lib():
         l.ora_upper_list.argtypes = [P, I64, I32]
         l.ora_search.restype = I32
         l.ora_search.argtypes = [P, P, I64, I32, I32, I32, I32, P, P, P, P]
+        l.ora_search_beam.restype = I32
+        l.ora_search_beam.argtypes = [P, P, I64, I32, I32, I32, I32, P, P, P, P]
         l.ora_dist.restype = C.c_float
         l.ora_dist.argtypes = [I32, P, P, I64]
         l.ora_normalize.argtypes = [P, I64]
@@ -92,6 +94,24 @@ class OracleIndex:
         st = Stats()
         rc = lib().ora_search(self._h, _p(q), nq, k, ef, ALGO[algo], num_threads,
                               _p(ids), _p(dists), _p(counts), C.byref(st))
+        if rc:
+            raise RuntimeError(lib().ora_last_error().decode())
+        if with_stats:
+            return ids, dists, counts, {f: getattr(st, f) for f, _ in Stats._fields_}
+        return ids, dists, counts
+
+    def search_beam(self, q, k: int, ef: int, beam: int, num_threads: int = 1,
+                    with_stats: bool = False):
+        """Model of the product's beam mode (FP32 vectors, `beam` expansions
+        per step, ties by internal id); no reference counterpart."""
+        q = np.ascontiguousarray(q, dtype=np.float32).reshape(-1, self.dim)
+        nq = q.shape[0]
+        ids = np.empty((nq, k), np.int32)
+        dists = np.empty((nq, k), np.float32)
+        counts = np.empty(nq, np.int32)
+        st = Stats()
+        rc = lib().ora_search_beam(self._h, _p(q), nq, k, ef, beam, num_threads,
+                                   _p(ids), _p(dists), _p(counts), C.byref(st))
         if rc:
             raise RuntimeError(lib().ora_last_error().decode())
         if with_stats:

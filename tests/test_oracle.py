@@ -148,6 +148,53 @@ def test_recall_of_config1_grows_with_ef():
     assert rec[-1] > 0.9
 
 
+@pytest.mark.parametrize("name", ["u128_l2", "dec100_cos", "gmm96_nd"])
+@pytest.mark.parametrize("k,ef", [(10, 64), (100, 32), (10, 300)])
+def test_beam_model_with_one_expansion_is_v1merge(name, k, ef):
+    # With one expansion per step and no tied distances, the beam model is
+    # SearchV1Merge step for step: same results and the same work counters.
+    ix = O.OracleIndex(F.index_path(name))
+    q = F.results(name)["queries"][1:]  # query 0 of the cosine set is all zeros (all ties)
+    a = ix.search(q, k, ef, "v1merge", with_stats=True)
+    b = ix.search_beam(q, k, ef, 1, with_stats=True)
+    for x, y in zip(a[:3], b[:3]):
+        np.testing.assert_array_equal(x, y)
+    assert a[3] == b[3]
+
+
+@pytest.mark.parametrize("beam", [1, 4, 16])
+def test_beam_model_tiny_graph_with_ef_at_least_n_is_exact(beam):
+    name = "tiny_l2"
+    ix = O.OracleIndex(F.index_path(name))
+    x = F.base_vectors(name)
+    q = F.results(name)["queries"]
+    ids, dists, _ = ix.search_beam(q, 10, 200, beam)
+    gt, gd = O.brute_force(x, q, 10, "l2")
+    np.testing.assert_array_equal(ids, gt)
+    np.testing.assert_allclose(dists, gd, rtol=1e-6)
+
+
+@pytest.mark.parametrize("beam", [2, 8, 16])
+def test_beam_model_keeps_quality(beam):
+    name = "u128_l2"
+    ix = O.OracleIndex(F.index_path(name))
+    x = F.base_vectors(name)
+    q = F.results(name)["queries"]
+    ex_ids, _, _ = ix.search(q, 10, 64, "v1merge")
+    ids, dists, counts, st = ix.search_beam(q, 10, 64, beam, with_stats=True)
+    assert np.all(counts == 10)
+    assert np.all(np.diff(dists, axis=1) >= 0)
+    for i in range(len(q)):
+        assert len(set(ids[i].tolist())) == 10
+        for j in (0, 9):
+            assert dists[i, j] == O.dist(ix.func, q[i], x[ids[i, j]])
+    gt, gd = O.brute_force(x, q, 100, "l2")
+    r_ex = F.recall_tie_aware(ex_ids, gt, gd, 10)
+    r_b = F.recall_tie_aware(ids, gt, gd, 10)
+    assert r_b >= r_ex - 0.01, (r_b, r_ex)
+    assert st["expansions"] >= 64 * len(q)
+
+
 def test_stats_counts_are_consistent():
     ix = O.OracleIndex(F.index_path("u128_l2"))
     q = F.results("u128_l2")["queries"]
