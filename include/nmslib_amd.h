@@ -97,8 +97,9 @@ int nmsamd_set_precision(nmsamd_index* h, int storage, int rerank_mult);
 /* Search mode: 0 = the reference's SearchV1Merge replayed exactly (default);
  * 1..16 = beam mode, that many unexpanded items expanded per step (at most
  * 512 / maxM0) with ties ordered by internal id (no reference counterpart;
- * same array, capacity and stop rule, fewer serial steps).  SearchOld is
- * unaffected. */
+ * same array, capacity and stop rule, fewer serial steps).  Beam mode runs
+ * every query of a batch, whatever its ef (the hybrid switch to SearchOld at
+ * ef >= 1000 does not apply); algoType=old still runs SearchOld. */
 int nmsamd_set_beam(nmsamd_index* h, int beam);
 
 /* Per-call counters of the last search (expansions, level-0 distance
@@ -117,10 +118,10 @@ int nmsamd_collect_stats(nmsamd_index* h, int on);
 int nmsamd_last_query_us(const nmsamd_index* h, float* out_us, int64_t nq);
 
 /* Beam mode, collection on: device time of the last search per phase of a
- * step (pick, adjacency + visited, distances, filter, sort + merge, advance),
- * in microseconds summed over queries.  Diagnostic; no reference
- * counterpart. */
-int nmsamd_last_phase_us(const nmsamd_index* h, double* out6);
+ * step (pick, adjacency + visited, distances, filter, sort + next-pick
+ * prediction, rank search, rebuild of the array, advance), in microseconds
+ * summed over queries.  Diagnostic; no reference counterpart. */
+int nmsamd_last_phase_us(const nmsamd_index* h, double* out8);
 
 /* Exact k-NN over the loaded index's FP32 vectors on MFMA: the device
  * counterpart of the reference's seq_search method (method/seqsearch.cc:

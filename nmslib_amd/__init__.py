@@ -199,9 +199,10 @@ class FloatIndex:
     def lastPhaseUs(self):
         """Beam mode with collectStats(True): device time per step phase, us
         summed over the queries of the last search."""
-        out = np.zeros(6, np.float64)
+        out = np.zeros(8, np.float64)
         _check(load_library().nmsamd_last_phase_us(self._h, _ptr(out)))
-        keys = ("pick", "adjacency_visited", "distances", "filter", "sort_merge", "advance")
+        keys = ("pick", "adjacency_visited", "distances", "filter", "sort_predict",
+                "rank_search", "rebuild", "advance")
         return {k: float(v) for k, v in zip(keys, out)}
 
     def lastQueryTimesUs(self, nq: int):

@@ -61,8 +61,9 @@ struct SearchStats {
   unsigned long long list_ints_read;
   unsigned long long tie_paths;
   // beam mode: 100 MHz ticks summed over queries per phase of a step
-  // (pick, adjacency+visited, distances, filter, sort+merge, advance)
-  unsigned long long phase_ticks[6];
+  // (pick, adjacency+visited, distances, filter, sort+prediction, rank search,
+  // rebuild, advance)
+  unsigned long long phase_ticks[8];
 };
 
 struct SearchArgs {

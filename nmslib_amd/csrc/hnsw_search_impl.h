@@ -440,6 +440,9 @@ struct WaveLds {
   uint2* items;    // Y [ent] beam step: admitted items, then the same sorted
   uint4* meta;     // [cap / 64 + 2] beam merge: per chunk, which positions new
                    // items take (x, y) and how many land below it (z)
+  uint2* pfa;      // [24] beam step: first unexpanded items of the array
+  int* pfpos;      // [24] their positions
+  uint2* pfq;      // [16] the next step's picks, claimed in advance
 };
 
 __device__ __forceinline__ WaveLds carve(unsigned char* base, int cap, int dstride, int ent) {
@@ -458,6 +461,9 @@ __device__ __forceinline__ WaveLds carve(unsigned char* base, int cap, int dstri
   w.bid = y + ent;
   w.items = reinterpret_cast<uint2*>(y);
   w.meta = reinterpret_cast<uint4*>(y + 2 * ent);
+  w.pfa = reinterpret_cast<uint2*>(w.meta + (cap / 64 + 2));
+  w.pfq = w.pfa + 24;
+  w.pfpos = reinterpret_cast<int*>(w.pfq + 16);
   return w;
 }
 
@@ -569,14 +575,22 @@ __device__ __forceinline__ void greedy_descent(const DevIndex& ix, const float* 
   }
 }
 
-// position of the first item at or after p that is not expanded yet (n if none)
+// position of the first item at or after p that is not expanded yet (n if
+// none); four chunks of 64 per round trip
 __device__ __forceinline__ int first_unused(const uint2* A, int p, int n, int lane) {
   while (p < n) {
-    const int i = p + lane;
-    const bool unused = i < n && !(A[i].y & kUsedBit);
-    const uint64_t um = __ballot(unused);
-    if (um) return p + __ffsll((long long)um) - 1;
-    p = min(p + 64, n);
+    uint32_t y[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = p + 64 * u + lane;
+      y[u] = i < n ? A[i].y : kUsedBit;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint64_t um = __ballot(!(y[u] & kUsedBit));
+      if (um) return p + 64 * u + __ffsll((long long)um) - 1;
+    }
+    p += 256;
   }
   return n;
 }
@@ -588,7 +602,7 @@ struct QueryCounters {
 // Optional FP32 rerank, then the first min(k, n) items of the sorted array in
 // (distance, internal id) order, and the query's counters.
 template <int D, int FUNC, int ST>
-__device__ void finish_query(const SearchArgs& a, const WaveLds& w, int n, int qi, int k,
+__device__ __forceinline__ void finish_query(const SearchArgs& a, const WaveLds& w, int n, int qi, int k,
                              int lane, int h, const QueryCounters& c) {
   const DevIndex& ix = a.ix;
   uint2* A = w.A;
@@ -665,7 +679,7 @@ __device__ void finish_query(const SearchArgs& a, const WaveLds& w, int n, int q
 // ---------------------------------------------------------------------------
 // SearchV1Merge at level 0 for one query held by one wave.
 template <int D, int FUNC, int ST>
-__device__ void v1merge_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, uint8_t ep,
+__device__ __forceinline__ void v1merge_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, uint8_t ep,
                               int qi, int k, int ef, int lane, int h, uint64_t ltm) {
   const DevIndex& ix = a.ix;
   const int C = max(ef, k);
@@ -912,8 +926,10 @@ __device__ __forceinline__ int lower_count_kid(const uint2* A, int n, float v, u
 
 // The value lane (lane ^ d) holds.  d = 1, 2: DPP quad permutes; 4, 8: DPP
 // row shifts (row_shl:d brings lane i + d, row_shr:d lane i - d, within rows
-// of 16); 16: ds_swizzle in bit mode (and 0x1f, xor 16); 32: a permute
-// through the LDS crossbar.  d is a constant after unrolling.
+// of 16); 16, 32: v_permlane16_swap / v_permlane32_swap with both operands v,
+// which return {v with its odd rows (upper half) replaced by the even rows
+// (lower half), v with its even rows (lower half) replaced by the odd rows
+// (upper half)}.  d is a constant after unrolling.
 __device__ __forceinline__ uint32_t xor_lane(uint32_t v, int d, int lane) {
   switch (d) {
     case 1: return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
@@ -928,8 +944,14 @@ __device__ __forceinline__ uint32_t xor_lane(uint32_t v, int d, int lane) {
       const int dn = __builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);
       return (uint32_t)((lane & 8) ? dn : up);
     }
-    case 16: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (16 << 10));
-    default: return (uint32_t)__shfl_xor((int)v, d);
+    case 16: {
+      const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+      return (lane & 16) ? r[0] : r[1];
+    }
+    default: {
+      const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+      return (lane & 32) ? r[0] : r[1];
+    }
   }
 }
 
@@ -992,9 +1014,16 @@ __device__ __forceinline__ void bitonic_sort(float (&k)[R], uint32_t (&id)[R], i
 // below p}].  Output chunk c reads only below 64 (c + 1) and writes
 // [64 c, 64 (c + 1)), so going top-down the reads of lower chunks may run ahead
 // of the writes: eight chunks go in flight at once, with no lane exchange.
-template <int R>
+// after_sort() runs once X is sorted and A not yet changed.
+//
+// When after_sort() leaves nxt = {pa, kx}: the first unexpanded item of the
+// merged array is either old A[pa] (pa < 0: none) or sorted new item kx (kx >=
+// m: none), both unclaimed; its position goes to *first (nout when neither
+// survives): A[pa] moves up by #{j : L_j <= pa}, item j lands at L_j + j.
+template <int R, class F, class G>
 __device__ __forceinline__ int sort_merge(uint2* A, int n_, int C_, uint2* X, int m_,
-                                          uint4* meta, int lane) {
+                                          uint4* meta, int lane, F&& after_sort, G&& lap,
+                                          const int2& nxt, int* first) {
   const int n = __builtin_amdgcn_readfirstlane(n_);
   const int C = __builtin_amdgcn_readfirstlane(C_);
   const int m = __builtin_amdgcn_readfirstlane(m_);
@@ -1019,6 +1048,9 @@ __device__ __forceinline__ int sort_merge(uint2* A, int n_, int C_, uint2* X, in
     const int j = r * 64 + lane;
     if (j < m) X[j] = make_uint2(__float_as_uint(k[r]), id[r]);
   }
+  WSYNC();
+  after_sort();
+  lap(4);
   int L[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) L[r] = 0;
@@ -1033,8 +1065,28 @@ __device__ __forceinline__ int sort_merge(uint2* A, int n_, int C_, uint2* X, in
     }
   }
   const int nout = min(C, n + m);
+  if (first) {
+    const int pa = __builtin_amdgcn_readfirstlane(nxt.x);
+    const int kx = __builtin_amdgcn_readfirstlane(nxt.y);
+    int fa = nout, fx = nout;
+    if (pa >= 0) {
+      int below = 0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) below += __popcll(__ballot(r * 64 + lane < m && L[r] <= pa));
+      fa = pa + below;
+    }
+    if (kx < m) {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if ((kx >> 6) == r) fx = __builtin_amdgcn_readlane(L[r], kx & 63) + kx;
+    }
+    *first = min(nout, min(fa, fx));
+  }
   const int P0 = __builtin_amdgcn_readlane(L[0], 0);
-  if (P0 >= nout) return P0;
+  if (P0 >= nout) {
+    lap(5);
+    return P0;
+  }
   const int clo = P0 >> 6, chi = (nout - 1) >> 6;
   for (int c = lane; c <= chi - clo; c += 64) meta[c] = make_uint4(0u, 0u, 0u, 0u);
   WSYNC();
@@ -1066,6 +1118,7 @@ __device__ __forceinline__ int sort_merge(uint2* A, int n_, int C_, uint2* X, in
     base += tot;
   }
   WSYNC();
+  lap(5);
   for (int c = chi; c >= clo; c -= 8) {
     uint4 md[8];
 #pragma unroll
@@ -1087,11 +1140,12 @@ __device__ __forceinline__ int sort_merge(uint2* A, int n_, int C_, uint2* X, in
     for (int u = 0; u < 8; ++u)
       if (pos[u] >= 0) A[pos[u]] = v[u];
   }
+  lap(6);
   return P0;
 }
 
 template <int D, int FUNC, int ST, int WIDE>
-__device__ void beam_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, uint8_t ep,
+__device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, uint8_t ep,
                            int qi, int k, int ef, int lane, int h, uint64_t ltm) {
   const DevIndex& ix = a.ix;
   const int C = max(ef, k);
@@ -1099,6 +1153,9 @@ __device__ void beam_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, 
   const int W = max(1, min(a.beam, E / ix.maxM0));
   const int erows = E >> 6;
   const uint32_t hslots = 2u * (uint32_t)E;
+  // e / maxM0 = (e * mdiv) >> 16 exactly for e < 65536 / maxM0 (here e < 512,
+  // maxM0 <= 64): the rounding error of mdiv adds less than e / 65536 < 1 / maxM0
+  const int mdiv = (65536 + ix.maxM0 - 1) / ix.maxM0;
   uint2* A = w.A;
   unsigned long long nexp = 0, nd0 = 0, ndu = 1, nl = 0;
 
@@ -1113,35 +1170,129 @@ __device__ void beam_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, 
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   int ce = 0;
-  // per-phase device time (100 MHz ticks) when counters are collected
+  // per-phase device time (100 MHz ticks) when counters are collected:
+  // pick, adjacency + visited, distances, filter, sort + prediction, rank
+  // search, rebuild, advance
   const bool timed = a.stats != nullptr;
-  unsigned long long ph[6] = {0, 0, 0, 0, 0, 0};
+  unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   uint64_t tmark = timed ? __builtin_amdgcn_s_memrealtime() : 0;
-  auto lap = [&](int p) {
+  auto lap = [&](int p) __attribute__((always_inline)) {
     if (timed) {
       const uint64_t now = __builtin_amdgcn_s_memrealtime();
       ph[p] += now - tmark;
       tmark = now;
     }
   };
-  while (ce < min(n, ef)) {
-    WSYNC();
-    // the first W unexpanded items below min(n, ef), in position order
-    const int lim = min(n, ef);
-    int nsel = 0;
-    for (int p = ce; p < lim && nsel < W; p += 64) {
-      const int i = p + lane;
-      const bool un = i < lim && !(A[i].y & kUsedBit);
-      const uint64_t um = __ballot(un);
-      const int take = min(__popcll(um), W - nsel);
-      const int r = __popcll(um & ltm);
-      if (un && r < take) {
-        const uint32_t y = A[i].y;
-        A[i].y = y | kUsedBit;
-        w.picks[nsel + r] = (int)y;
+  // The next step's picks are the W smallest unexpanded items once this
+  // step's admitted items are in: with capacity C = ef the survivors of the
+  // truncation are a prefix of the array, so they are the W smallest of (the
+  // first W unexpanded items of the array) and (the W smallest admitted
+  // items), cut at the array's new last item.  As soon as the admitted items
+  // are sorted, those W are claimed (marked expanded in place, in order in
+  // pfq) and their adjacency rows (tn) load while the merge runs; their
+  // visited marks (vn) load right after it.  The next step then takes the
+  // surviving prefix of pfq without scanning the array.
+  const bool claim = C == ef;
+  int pf_n = 0;  // > 0: pfq[0..pf_n) claimed for the next step
+  int2 nxt = make_int2(-1, 0);  // first unclaimed items (see sort_merge)
+  int first = 0;
+  int tn[8], vn[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) tn[r] = vn[r] = -1;
+  auto predict = [&](uint2* Xs, int nx) __attribute__((always_inline)) {
+    int na = 0;  // up to W + 1: the first one left unclaimed is the next to expand
+    for (int p = ce; p < n && na <= W; p += 128) {
+      uint2 it[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int i = p + 64 * u + lane;
+        it[u] = i < n ? A[i] : make_uint2(0u, kUsedBit);
       }
-      nsel += take;
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const bool un = !(it[u].y & kUsedBit);
+        const uint64_t um = __ballot(un);
+        const int r = na + __popcll(um & ltm);
+        if (un && r <= W) {
+          w.pfa[r] = it[u];
+          w.pfpos[r] = p + 64 * u + lane;
+        }
+        na = min(W + 1, na + __popcll(um));
+      }
     }
+    WSYNC();
+    const int naw = min(W, na);
+    const int nxw = min(W, nx);
+    const int tot = naw + nxw;
+    float ck = kInf();
+    uint32_t ci = 0xffffffffu;
+    if (lane < naw) {
+      const uint2 v = w.pfa[lane];
+      ck = __uint_as_float(v.x);
+      ci = v.y;
+    } else if (lane < tot) {
+      const uint2 v = Xs[lane - naw];
+      ck = __uint_as_float(v.x);
+      ci = v.y;
+    }
+    int rank = 0;
+    for (int j = 0; j < tot; ++j) {
+      const float kj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ck), j));
+      const uint32_t ij = (uint32_t)__builtin_amdgcn_readlane((int)ci, j);
+      rank += kid_less(kj, ij, ck, ci);
+    }
+    const bool take = lane < tot && rank < W;
+    if (take) {
+      w.pfq[rank] = make_uint2(__float_as_uint(ck), ci);
+      if (lane < naw) A[w.pfpos[lane]].y = ci | kUsedBit;
+      else Xs[lane - naw].y = ci | kUsedBit;
+    }
+    pf_n = min(W, tot);
+    // the claimed items of each side are a prefix of it
+    const int ka = __popcll(__ballot(take && lane < naw));
+    nxt = make_int2(ka < na ? w.pfpos[ka] : -1, pf_n - ka);
+    WSYNC();
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      tn[r] = -1;
+      if (r < erows) {
+        const int e = lane + 64 * r;
+        const int li = (e * mdiv) >> 16;
+        const int sl = e - li * ix.maxM0;
+        if (li < pf_n) tn[r] = ix.l0[(int64_t)w.pfq[li].y * ix.l0stride + sl];
+      }
+    }
+  };
+  auto no_predict = [] {};
+  for (;;) {
+    WSYNC();
+    int nsel = 0;
+    const bool claimed = pf_n > 0;
+    if (claimed) {
+      // the claimed picks that survived the last merge (a prefix of pfq)
+      const uint2 last = A[n - 1];
+      const uint2 q = lane < pf_n ? w.pfq[lane] : make_uint2(0u, 0u);
+      const bool surv = lane < pf_n && !kid_less(__uint_as_float(last.x), last.y & ~kUsedBit,
+                                                 __uint_as_float(q.x), q.y);
+      nsel = __popcll(__ballot(surv));
+    } else {
+      // the first W unexpanded items below min(n, ef), in position order
+      const int lim = min(n, ef);
+      for (int p = ce; p < lim && nsel < W; p += 64) {
+        const int i = p + lane;
+        const bool un = i < lim && !(A[i].y & kUsedBit);
+        const uint64_t um = __ballot(un);
+        const int take = min(__popcll(um), W - nsel);
+        const int r = __popcll(um & ltm);
+        if (un && r < take) {
+          const uint32_t y = A[i].y;
+          A[i].y = y | kUsedBit;
+          w.picks[nsel + r] = (int)y;
+        }
+        nsel += take;
+      }
+    }
+    if (nsel == 0) break;
     nexp += nsel;
     const float topk = akey(A, n - 1);
     const int nsnap = n;
@@ -1150,49 +1301,59 @@ __device__ void beam_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, 
     WSYNC();
     lap(0);
 
-    // adjacency rows of the picks, entry e = 64 r + lane (W * maxM0 <= E)
-    int t[8];
+    // adjacency rows of the picks, entry e = 64 r + lane (W * maxM0 <= E),
+    // and the visited marks of their entries: prefetched for claimed picks,
+    // else loaded now, every row's loads issued before any is used (an id of
+    // -1 reads mark 0 and is ignored)
+    int t[8], vb[8];
+    if (claimed) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int li = ((lane + 64 * r) * mdiv) >> 16;
+        t[r] = li < nsel ? tn[r] : -1;
+        vb[r] = vn[r];
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        t[r] = -1;
+        if (r < erows) {
+          const int e = lane + 64 * r;
+          const int li = (e * mdiv) >> 16;
+          const int sl = e - li * ix.maxM0;
+          if (li < nsel) t[r] = ix.l0[(int64_t)w.picks[li] * ix.l0stride + sl];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 8; ++r) vb[r] = (int)vis[t[r] >= 0 ? t[r] : 0];
+    }
     bool fresh[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-      t[r] = -1;
-      if (r < erows) {
-        const int e = lane + 64 * r;
-        const int li = e / ix.maxM0;
-        const int sl = e - li * ix.maxM0;
-        if (li < nsel) t[r] = ix.l0[(int64_t)w.picks[li] * ix.l0stride + sl];
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      fresh[r] = false;
-      if (r < erows) {
-        const bool valid = t[r] >= 0;
-        nl += __popcll(__ballot(valid));
-        fresh[r] = valid && vis[t[r]] != ep;
-      }
+      const bool valid = r < erows && t[r] >= 0;
+      nl += __popcll(__ballot(valid));
+      fresh[r] = valid && vb[r] != (int)ep;
     }
     nl += nsel;
     // a neighbour shared by two picks is taken once (LDS hash, linear probing)
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-      if (r < erows) {
-        if (fresh[r]) {
-          uint32_t slot =
-              (uint32_t)(((uint64_t)((uint32_t)t[r] * 2654435761u) * hslots) >> 32);
-          for (;;) {
-            const uint32_t old = atomicCAS(&w.htab[slot], ~0u, (uint32_t)t[r]);
-            if (old == ~0u) break;
-            if (old == (uint32_t)t[r]) {
-              fresh[r] = false;
-              break;
-            }
-            if (++slot == hslots) slot = 0;
+      if (r < erows && fresh[r]) {
+        uint32_t slot = (uint32_t)(((uint64_t)((uint32_t)t[r] * 2654435761u) * hslots) >> 32);
+        for (;;) {
+          const uint32_t old = atomicCAS(&w.htab[slot], ~0u, (uint32_t)t[r]);
+          if (old == ~0u) break;
+          if (old == (uint32_t)t[r]) {
+            fresh[r] = false;
+            break;
           }
+          if (++slot == hslots) slot = 0;
         }
-        if (fresh[r]) vis[t[r]] = ep;
       }
     }
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+      if (fresh[r]) vis[t[r]] = ep;
     int ncand = 0;
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
@@ -1220,29 +1381,48 @@ __device__ void beam_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, 
     }
     WSYNC();
     lap(3);
+    pf_n = 0;
     if (m > 0) {
       // sort by (distance, id) and merge, at most 256 items per pass (a later
-      // pass only holds items the earlier ones do not)
-      for (int b = 0; b < m; b += 256) {
-        const int cnt = min(256, m - b);
+      // pass only holds items the earlier ones do not); a single pass also
+      // claims the next picks
+      if (m <= 256 && claim) {
+        auto pred = [&]() __attribute__((always_inline)) { predict(w.items, m); };
         int L0;
-        if (cnt <= 64) L0 = sort_merge<1>(A, n, C, w.items + b, cnt, w.meta, lane);
-        else if (cnt <= 128) L0 = sort_merge<2>(A, n, C, w.items + b, cnt, w.meta, lane);
-        else L0 = sort_merge<4>(A, n, C, w.items + b, cnt, w.meta, lane);
-        n = min(C, n + cnt);
-        if (L0 < ce) ce = L0;
-        WSYNC();
+        if (m <= 64) L0 = sort_merge<1>(A, n, C, w.items, m, w.meta, lane, pred, lap, nxt, &first);
+        else if (m <= 128) L0 = sort_merge<2>(A, n, C, w.items, m, w.meta, lane, pred, lap, nxt, &first);
+        else L0 = sort_merge<4>(A, n, C, w.items, m, w.meta, lane, pred, lap, nxt, &first);
+        n = min(C, n + m);
+        ce = first;
+      } else {
+        for (int b = 0; b < m; b += 256) {
+          const int cnt = min(256, m - b);
+          int L0;
+          if (cnt <= 64) L0 = sort_merge<1>(A, n, C, w.items + b, cnt, w.meta, lane, no_predict, lap, nxt, nullptr);
+          else if (cnt <= 128) L0 = sort_merge<2>(A, n, C, w.items + b, cnt, w.meta, lane, no_predict, lap, nxt, nullptr);
+          else L0 = sort_merge<4>(A, n, C, w.items + b, cnt, w.meta, lane, no_predict, lap, nxt, nullptr);
+          n = min(C, n + cnt);
+          if (L0 < ce) ce = L0;
+          WSYNC();
+        }
       }
+    } else if (claim) {
+      predict(w.items, 0);
+      ce = nxt.x >= 0 ? nxt.x : n;
       lap(4);
     }
     WSYNC();
-    ce = first_unused(A, ce, n, lane);
+    // this step's marks land before any later read of them, then the marks of
+    // the predicted picks' neighbours load while the next picks are found
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    lap(5);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) vn[r] = (int)vis[tn[r] >= 0 ? tn[r] : 0];
+    if (pf_n == 0) ce = first_unused(A, ce, n, lane);
+    lap(7);
   }
   WSYNC();
   if (timed && lane == 0)
-    for (int p = 0; p < 6; ++p) atomicAdd(&a.stats->phase_ticks[p], ph[p]);
+    for (int p = 0; p < 8; ++p) atomicAdd(&a.stats->phase_ticks[p], ph[p]);
   const QueryCounters c{nexp, nd0, ndu, nl, 0};
   finish_query<D, FUNC, ST>(a, w, n, qi, k, lane, h, c);
 }
@@ -1257,7 +1437,7 @@ __device__ void beam_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, 
 // sorted-array space holds the ef-heap in [0, ef] and the result queue
 // (sorted by (distance, internal id), the order KNNQueue pops in) after it.
 template <int D, int FUNC, int ST>
-__device__ void old_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, uint8_t ep, int qi,
+__device__ __forceinline__ void old_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, uint8_t ep, int qi,
                           int k, int ef, int lane, int h, uint64_t ltm, int64_t slot) {
   const DevIndex& ix = a.ix;
   unsigned long long nexp = 0, nd0 = 0, ndu = 1, nl = 0;
@@ -1394,15 +1574,14 @@ __device__ void old_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, u
   }
 }
 
-// MODE = kModeAll runs every search mode.  kModeBeam / kModeBeamWide hold
-// only the beam search (launched when no query of the batch takes SearchOld),
-// so its loop does not carry the exact search's register state;
-// kModeBeamWide keeps a round's vector loads in flight together (~240 VGPRs,
-// one wave per SIMD) and is used when the LDS footprint already limits
-// residency to about a wave per SIMD.
+// MODE = kModeAll runs the reference's searches (V1Merge, SearchOld by
+// algoType / ef).  kModeBeam / kModeBeamWide run the beam search for every
+// query of the batch, each kernel carrying only its own loop's register
+// state; kModeBeamWide keeps a round's vector loads in flight together (one
+// wave per SIMD) and is used when the LDS footprint already limits residency
+// to about a wave per SIMD.
 template <int D, int FUNC, int ST, int MODE>
-__global__ void __launch_bounds__(kWave* kWavesPerBlock)
-    hnsw_search_kernel(SearchArgs a, int lds_wave_bytes) {
+__device__ __forceinline__ void search_body(const SearchArgs& a, int lds_wave_bytes) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -1468,8 +1647,6 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock)
       const bool use_old = a.algo == kAlgoOld || (a.algo == kAlgoHybrid && ef >= 1000);
       if (use_old)
         old_query<D, FUNC, ST>(a, w, vis, ep, qi, k, ef, lane, h, ltm, slot);
-      else if (a.beam > 0)
-        beam_query<D, FUNC, ST, 0>(a, w, vis, ep, qi, k, ef, lane, h, ltm);
       else
         v1merge_query<D, FUNC, ST>(a, w, vis, ep, qi, k, ef, lane, h, ltm);
     } else {
@@ -1479,6 +1656,28 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock)
     if (a.q_ticks && lane == 0)
       a.q_ticks[qi] = (uint32_t)(__builtin_amdgcn_s_memrealtime() - t_start);
   }
+}
+
+// kModeAll / kModeBeam: register budget of two waves per SIMD
+template <int D, int FUNC, int ST, int MODE>
+__global__ void __launch_bounds__(kWave* kWavesPerBlock) __attribute__((amdgpu_waves_per_eu(2)))
+    hnsw_search_kernel(SearchArgs a, int lds_wave_bytes) {
+  search_body<D, FUNC, ST, MODE>(a, lds_wave_bytes);
+}
+
+// kModeBeamWide: the whole register file of one wave per SIMD
+template <int D, int FUNC, int ST>
+__global__ void __launch_bounds__(kWave* kWavesPerBlock)
+    hnsw_search_kernel_wide(SearchArgs a, int lds_wave_bytes) {
+  search_body<D, FUNC, ST, kModeBeamWide>(a, lds_wave_bytes);
+}
+
+using KernelFn = void (*)(SearchArgs, int);
+
+template <int D, int FUNC, int ST, int MODE>
+KernelFn kernel_of() {
+  if constexpr (MODE == kModeBeamWide) return hnsw_search_kernel_wide<D, FUNC, ST>;
+  else return hnsw_search_kernel<D, FUNC, ST, MODE>;
 }
 
 }  // namespace nmsamd

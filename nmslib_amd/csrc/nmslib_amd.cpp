@@ -362,9 +362,9 @@ int step_entries(const nmsamd_index* h) {
 }
 
 // sorted array + query + two scratch regions of 8 * ent bytes + the merge's
-// 16-byte record per 64 positions
+// 16-byte record per 64 positions + the claimed next picks (416 bytes)
 int lds_wave_bytes(int cap, int dstride, int ent) {
-  const int b = ((cap + 1) & ~1) * 8 + dstride * 4 + 16 * ent + (cap / 64 + 2) * 16;
+  const int b = ((cap + 1) & ~1) * 8 + dstride * 4 + 16 * ent + (cap / 64 + 2) * 16 + 416;
   return (b + 15) / 16 * 16;
 }
 
@@ -374,8 +374,10 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
                 const int32_t* ef_of, int max_k, int max_ef, int32_t* ids, float* dists,
                 int32_t* counts, hipStream_t s) {
   if (!h->loaded) fail(NMSAMD_EINVAL, "Must call createIndex or loadIndex before this method");
-  // Hnsw::Search (hnsw.cc:724): SearchOld for algoType=old or hybrid with ef >= 1000
-  const bool any_old = h->algo == kOld || (h->algo == kHybrid && max_ef >= 1000);
+  // Hnsw::Search (hnsw.cc:724): SearchOld for algoType=old or hybrid with ef
+  // >= 1000; beam mode replaces V1Merge and the hybrid switch for the batch
+  const bool beam = h->beam > 0 && h->algo != kOld;
+  const bool any_old = !beam && (h->algo == kOld || (h->algo == kHybrid && max_ef >= 1000));
   int cap = std::max(1, std::max(max_ef, max_k));
   if (any_old) cap = std::max(cap, max_ef + 1 + max_k);
   const bool rerank = h->storage != kF32 && h->rerank_mult > 0;
@@ -383,7 +385,7 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
     const int r = std::min(std::max(max_ef, max_k), std::max(max_k, h->rerank_mult * max_k));
     cap = std::max(cap, std::max(max_ef, max_k) + r);
   }
-  const int ent = step_entries(h);
+  const int ent = beam ? step_entries(h) : 128;
   const int lds = lds_wave_bytes(cap, h->dstride, ent);
   if ((size_t)lds > kLdsPerCu)
     fail(NMSAMD_EINVAL, "efSearch / k too large: the search array of " + std::to_string(cap) +
@@ -445,11 +447,10 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   a.old_heap_id = h->d_oldi.p;
   a.old_heap_cap = heap_cap;
   hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
-  // a beam batch with no SearchOld query takes the beam-only kernel, which
-  // keeps a round's gathers in flight together when the LDS footprint
-  // already holds residency to about one wave per SIMD
-  const int mode = (h->beam > 0 && !any_old)
-                       ? (wpb * per_cu <= 8 ? kModeBeamWide : kModeBeam) : kModeAll;
+  // beam batches take the beam-only kernels; the wide one keeps a round's
+  // gathers in flight together when the LDS footprint already holds
+  // residency to about one wave per SIMD
+  const int mode = beam ? (wpb * per_cu <= 8 ? kModeBeamWide : kModeBeam) : kModeAll;
   hip_check(launch_search(a, blocks, wpb, lds, mode, s), "search kernel launch");
   hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
   h->pending = true;
@@ -787,10 +788,10 @@ int nmsamd_last_query_us(const nmsamd_index* h, float* out_us, int64_t nq) {
   });
 }
 
-int nmsamd_last_phase_us(const nmsamd_index* h, double* out6) {
+int nmsamd_last_phase_us(const nmsamd_index* h, double* out8) {
   return guarded([&] {
-    if (!h || !out6) fail(NMSAMD_EINVAL, "null argument");
-    for (int p = 0; p < 6; ++p) out6[p] = h->last_stats.phase_ticks[p] * 0.01;  // 100 MHz
+    if (!h || !out8) fail(NMSAMD_EINVAL, "null argument");
+    for (int p = 0; p < 8; ++p) out8[p] = h->last_stats.phase_ticks[p] * 0.01;  // 100 MHz
   });
 }
 

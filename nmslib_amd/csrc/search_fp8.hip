@@ -3,15 +3,13 @@
 
 namespace nmsamd {
 
-using KernelFn = void (*)(SearchArgs, int);
-
 KernelFn pick_kernel_fp8(int dim, int func, int mode) {
 #define NMS_CASE(D, W)                                                 \
   switch (func) {                                                      \
-    case kL2Sqr16: return hnsw_search_kernel<D, kL2Sqr16, kFP8, W>;      \
-    case kL2Sqr: return hnsw_search_kernel<D, kL2Sqr, kFP8, W>;          \
-    case kNormCos: return hnsw_search_kernel<D, kNormCos, kFP8, W>;      \
-    case kNegDot: return hnsw_search_kernel<D, kNegDot, kFP8, W>;        \
+    case kL2Sqr16: return kernel_of<D, kL2Sqr16, kFP8, W>();      \
+    case kL2Sqr: return kernel_of<D, kL2Sqr, kFP8, W>();          \
+    case kNormCos: return kernel_of<D, kNormCos, kFP8, W>();      \
+    case kNegDot: return kernel_of<D, kNegDot, kFP8, W>();        \
     default: return nullptr;                                           \
   }
 #define NMS_DIMS(W)                \
