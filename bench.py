@@ -212,6 +212,7 @@ def measure(idx, q, gids, gd, form, args, dev, log_prefix):
     step()
     torch.cuda.synchronize()
     st = idx.lastStats()
+    launch = idx.lastLaunch()
     lat = idx.lastQueryTimesUs(nq)
     phases = idx.lastPhaseUs() if beam else None
     idx.collectStats(False)
@@ -251,6 +252,7 @@ def measure(idx, q, gids, gd, form, args, dev, log_prefix):
                       "tie_path_expansions": round(st["tie_paths"] / nq, 1)},
         "alg_bytes_per_launch": int(alg_bytes),
         "achieved_gbs": round(alg_bytes / avg_kernel_s / 1e9, 1),
+        "launch": launch,
         "curve": curve,
     }
     if phases:
@@ -268,7 +270,7 @@ def main():
     ap.add_argument("--ef", type=int, default=0, help="fixed efSearch (0: smallest of the ladder "
                     "reaching recall@10 >= 0.95)")
     ap.add_argument("--algo", default="v1merge")
-    ap.add_argument("--forms", default="fp32-exact,bf16-beam4-rerank",
+    ap.add_argument("--forms", default="fp32-exact,fp32-beam16,bf16-beam16-rerank",
                     help=f"comma list of {sorted(FORMS)}; the headline is the fastest")
     ap.add_argument("--nq", type=int, default=10000)
     ap.add_argument("--sweep", action="store_true", help="also record the recall/QPS curve")
@@ -350,7 +352,7 @@ def main():
     if args.serve:
         rates = [float(r) for r in args.serve.split(",") if r]
         serve = serve_bench(idx, q, [("fp32 exact V1Merge", "fp32", 0, 0),
-                                     ("bf16 beam4 + fp32 rerank x2", "bf16", 2, 4)],
+                                     ("bf16 beam16 + fp32 rerank x2", "bf16", 2, 16)],
                             rates, args.serve_seconds)
 
     cpu = None
@@ -386,7 +388,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": head["achieved_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(head["achieved_gbs"] / HBM_PEAK_GBS, 4),
                      "traffic": _measured_traffic(args.traffic, head["form"], head["efSearch"]),
-                     "kernel": "hnsw_search_kernel",
+                     "kernel": head["launch"]["kernel"],
                      "avg_kernel_ms": head["avg_kernel_ms"],
                      "alg_bytes_per_launch": head["alg_bytes_per_launch"]},
         "cpu_baseline": cpu,

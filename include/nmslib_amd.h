@@ -123,6 +123,13 @@ int nmsamd_last_query_us(const nmsamd_index* h, float* out_us, int64_t nq);
  * summed over queries.  Diagnostic; no reference counterpart. */
 int nmsamd_last_phase_us(const nmsamd_index* h, double* out8);
 
+/* Shape of the last search launch: kernel variant (0 = hnsw_search_kernel
+ * with V1Merge / SearchOld, 1 = hnsw_search_kernel beam search, 2 =
+ * hnsw_search_kernel_wide beam search), blocks, waves per block, LDS bytes
+ * per wave, neighbour entries per step.  Diagnostic; no reference
+ * counterpart. */
+int nmsamd_last_launch(const nmsamd_index* h, int64_t* out5);
+
 /* Exact k-NN over the loaded index's FP32 vectors on MFMA: the device
  * counterpart of the reference's seq_search method (method/seqsearch.cc:
  * 143-177) and of its gold standard (gold_standard.h:151-175: ranked by

@@ -68,6 +68,7 @@ SIGNATURES = {
     "nmsamd_set_beam": (C.c_int, [_P, C.c_int]),
     "nmsamd_brute_force": (C.c_int, [_P, _P, _I64, _I64, _I32, _P, _P, _P]),
     "nmsamd_last_phase_us": (C.c_int, [_P, _P]),
+    "nmsamd_last_launch": (C.c_int, [_P, _P]),
 }
 
 
@@ -204,6 +205,16 @@ class FloatIndex:
         keys = ("pick", "adjacency_visited", "distances", "filter", "sort_predict",
                 "rank_search", "rebuild", "advance")
         return {k: float(v) for k, v in zip(keys, out)}
+
+    def lastLaunch(self):
+        """Shape of the last search launch (kernel variant, grid, LDS)."""
+        out = np.zeros(5, np.int64)
+        _check(load_library().nmsamd_last_launch(self._h, _ptr(out)))
+        mode = int(out[0])
+        return {"kernel": "hnsw_search_kernel_wide" if mode == 2 else "hnsw_search_kernel",
+                "mode": ("v1merge/old", "beam", "beam (wide)")[mode], "blocks": int(out[1]),
+                "waves_per_block": int(out[2]), "lds_bytes_per_wave": int(out[3]),
+                "entries_per_step": int(out[4])}
 
     def lastQueryTimesUs(self, nq: int):
         """Per-query device search time (us) of the last search made with

@@ -438,6 +438,7 @@ struct WaveLds {
   float* bkey;     // Y [ent] buffer (list order)
   int* bid;        // Y
   uint2* items;    // Y [ent] beam step: admitted items, then the same sorted
+  // beam mode only (not allocated for the other modes):
   uint4* meta;     // [cap / 64 + 2] beam merge: per chunk, which positions new
                    // items take (x, y) and how many land below it (z)
   uint2* pfa;      // [24] beam step: first unexpanded items of the array
@@ -576,8 +577,14 @@ __device__ __forceinline__ void greedy_descent(const DevIndex& ix, const float* 
 }
 
 // position of the first item at or after p that is not expanded yet (n if
-// none); four chunks of 64 per round trip
+// none): one chunk of 64, then four chunks per round trip
 __device__ __forceinline__ int first_unused(const uint2* A, int p, int n, int lane) {
+  if (p < n) {
+    const int i = p + lane;
+    const uint64_t um = __ballot(i < n && !(A[i].y & kUsedBit));
+    if (um) return p + __ffsll((long long)um) - 1;
+    p += 64;
+  }
   while (p < n) {
     uint32_t y[4];
 #pragma unroll

@@ -145,6 +145,7 @@ struct nmsamd_index {
   bool pending = false;
   int64_t last_nq = 0;
   SearchStats last_stats{};
+  int64_t last_launch[5] = {0, 0, 0, 0, 0};  // kernel mode, blocks, waves/block, LDS/wave, entries
   double last_ms = 0;
   std::mutex mu;  // one search at a time per handle
   int cu_count = 0;
@@ -361,10 +362,12 @@ int step_entries(const nmsamd_index* h) {
   return std::max(128, (w * (int)h->maxM0 + 63) / 64 * 64);
 }
 
-// sorted array + query + two scratch regions of 8 * ent bytes + the merge's
-// 16-byte record per 64 positions + the claimed next picks (416 bytes)
-int lds_wave_bytes(int cap, int dstride, int ent) {
-  const int b = ((cap + 1) & ~1) * 8 + dstride * 4 + 16 * ent + (cap / 64 + 2) * 16 + 416;
+// sorted array + query + two scratch regions of 8 * ent bytes; beam mode
+// adds the merge's 16-byte record per 64 positions and the claimed next picks
+// (416 bytes)
+int lds_wave_bytes(int cap, int dstride, int ent, bool beam) {
+  int b = ((cap + 1) & ~1) * 8 + dstride * 4 + 16 * ent;
+  if (beam) b += (cap / 64 + 2) * 16 + 416;
   return (b + 15) / 16 * 16;
 }
 
@@ -386,7 +389,7 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
     cap = std::max(cap, std::max(max_ef, max_k) + r);
   }
   const int ent = beam ? step_entries(h) : 128;
-  const int lds = lds_wave_bytes(cap, h->dstride, ent);
+  const int lds = lds_wave_bytes(cap, h->dstride, ent, beam);
   if ((size_t)lds > kLdsPerCu)
     fail(NMSAMD_EINVAL, "efSearch / k too large: the search array of " + std::to_string(cap) +
                             " items exceeds the LDS of one wave");
@@ -453,6 +456,11 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   const int mode = beam ? (wpb * per_cu <= 8 ? kModeBeamWide : kModeBeam) : kModeAll;
   hip_check(launch_search(a, blocks, wpb, lds, mode, s), "search kernel launch");
   hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
+  h->last_launch[0] = mode;
+  h->last_launch[1] = blocks;
+  h->last_launch[2] = wpb;
+  h->last_launch[3] = lds;
+  h->last_launch[4] = ent;
   h->pending = true;
 }
 
@@ -788,6 +796,13 @@ int nmsamd_last_query_us(const nmsamd_index* h, float* out_us, int64_t nq) {
   });
 }
 
+int nmsamd_last_launch(const nmsamd_index* h, int64_t* out5) {
+  return guarded([&] {
+    if (!h || !out5) fail(NMSAMD_EINVAL, "null argument");
+    for (int i = 0; i < 5; ++i) out5[i] = h->last_launch[i];
+  });
+}
+
 int nmsamd_last_phase_us(const nmsamd_index* h, double* out8) {
   return guarded([&] {
     if (!h || !out8) fail(NMSAMD_EINVAL, "null argument");
@@ -919,7 +934,7 @@ int nmsamd_server_open(nmsamd_index* h, int lanes, int max_queries, int max_ef, 
       s->cap = std::max(s->cap, std::max(max_ef, max_k) +
                                     std::min(std::max(max_ef, max_k), h->rerank_mult * max_k));
     s->ent = step_entries(h);
-    s->lds = lds_wave_bytes(s->cap, h->dstride, s->ent);
+    s->lds = lds_wave_bytes(s->cap, h->dstride, s->ent, h->beam > 0);
     if ((size_t)s->lds > kLdsPerCu) fail(NMSAMD_EINVAL, "max_ef / max_k exceed the LDS of one wave");
     ensure_lowp(h);
     s->wpb = (int)std::max<size_t>(1, std::min<size_t>(kWavesPerBlock, kLdsPerCu / s->lds));
