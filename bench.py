@@ -231,6 +231,11 @@ def measure(idx, q, gids, gd, form, args, dev, log_prefix):
         kernel_ms.append(idx.lastStats()["kernel_ms"])
     torch.cuda.synchronize()
     elapsed = time.time() - t0
+    # the same batch through host buffers (knnQueryBatch semantics: queries
+    # copied in, results copied out over PCIe); reported beside, never `value`
+    t1 = time.time()
+    idx.knnQueryBatchArrays(q, k)
+    host_qps = nq / (time.time() - t1)
 
     # algorithmic bytes of one launch: adjacency ints read, every distance
     # evaluation's vector, the rerank's FP32 vectors, queries in, results out
@@ -243,6 +248,7 @@ def measure(idx, q, gids, gd, form, args, dev, log_prefix):
     res = {
         "form": form, "efSearch": ef, "recall_at_10": round(recall, 4),
         "qps": round(nq * args.steps / elapsed, 1),
+        "qps_host_buffers": round(host_qps, 1),
         "ms_per_step": round(elapsed * 1000 / args.steps, 3),
         "avg_kernel_ms": round(avg_kernel_s * 1000, 3),
         "latency_us_p50": round(float(np.percentile(lat, 50)), 1),
