@@ -450,10 +450,11 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   a.old_heap_id = h->d_oldi.p;
   a.old_heap_cap = heap_cap;
   hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
-  // beam batches take the beam-only kernels; the wide one keeps a round's
-  // gathers in flight together when the LDS footprint already holds
-  // residency to about one wave per SIMD
-  const int mode = beam ? (wpb * per_cu <= 8 ? kModeBeamWide : kModeBeam) : kModeAll;
+  // beam batches take the beam-only kernels; the wide one (registers for one
+  // wave per SIMD) keeps a round's gathers in flight together and is chosen
+  // when the LDS footprint already limits residency to that, the other one
+  // holds two waves per SIMD
+  const int mode = beam ? (wpb * per_cu <= 4 ? kModeBeamWide : kModeBeam) : kModeAll;
   hip_check(launch_search(a, blocks, wpb, lds, mode, s), "search kernel launch");
   hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
   h->last_launch[0] = mode;
@@ -939,7 +940,7 @@ int nmsamd_server_open(nmsamd_index* h, int lanes, int max_queries, int max_ef, 
     ensure_lowp(h);
     s->wpb = (int)std::max<size_t>(1, std::min<size_t>(kWavesPerBlock, kLdsPerCu / s->lds));
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, kLdsPerCu / ((size_t)s->lds * s->wpb)));
-    s->wide = (h->beam > 0 && s->wpb * per_cu <= 8) ? 1 : 0;
+    s->wide = (h->beam > 0 && s->wpb * per_cu <= 4) ? 1 : 0;
     // the device's resident waves at this footprint, shared by the lanes
     const int64_t total_blocks = (int64_t)h->cu_count * per_cu;
     const int64_t blocks_per_lane = std::max<int64_t>(1, total_blocks / lanes);

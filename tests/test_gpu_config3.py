@@ -1,0 +1,102 @@
+"""The 2,000,000 x 100 cosinesimil case at full size: an index built by the
+reference nmslib (M=32, efConstruction=400; its graph ships as a pack under
+workloads/, the vectors are regenerated from their seed), one uneven batch of
+8,192 queries with k drawn from {1, 10, 100} and efSearch from 32..800.
+
+The device's exact mode equals the oracle row for row on a sample of the
+batch, beam mode equals the beam model on the same sample, and the recall of
+both against the device brute force is reported.  Skipped when the pack is
+absent (it is too large to keep in the tree)."""
+import os
+import tempfile
+import time
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.gpu_util import load
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PACK = os.path.join(ROOT, "workloads", "decay2m_M32_efc400.pack.npz")
+SAMPLE = slice(0, 8192, 128)  # 64 queries of the batch
+
+
+@pytest.fixture(scope="module")
+def cfg3():
+    if not os.path.exists(PACK):
+        pytest.skip("2M cosine workload pack not present")
+    from nmslib_amd import datasets as D
+    from nmslib_amd import indexpack
+    path = os.path.join(tempfile.mkdtemp(prefix="nmsamd_cfg3_"), "decay2m.hnsw")
+    indexpack.unpack(indexpack.load_pack(PACK), D.base("decay2m"), path)
+    q = D.queries("decay2m")
+    rng = np.random.default_rng(8)
+    ks = rng.choice(np.array([1, 10, 100], np.int32), size=len(q)).astype(np.int32)
+    efs = rng.integers(32, 801, size=len(q)).astype(np.int32)
+    idx = load("decay2m", path, "cosinesimil")
+    gt, gd, _ = idx.bruteForce(q, 100)
+    yield path, idx, q, ks, efs, gt
+    os.remove(path)
+
+
+def _recall(ids, ks, gt):
+    out = {}
+    for kk in (1, 10, 100):
+        sel = np.nonzero(ks == kk)[0]
+        hit = sum(len(set(ids[i, :kk].tolist()) & set(gt[i, :kk].tolist())) for i in sel)
+        out[kk] = hit / (kk * len(sel))
+    return out
+
+
+def _timed_uneven(idx, q, ks, efs):
+    idx.knnQueryBatchUneven(q, ks, efs)  # warm-up (first use builds reduced-precision copies)
+    t = time.time()
+    ids, dists, counts = idx.knnQueryBatchUneven(q, ks, efs)
+    wall = time.time() - t
+    return ids, dists, counts, idx.lastStats()["kernel_ms"], wall
+
+
+def test_cfg3_uneven_exact_equals_oracle(cfg3):
+    path, idx, q, ks, efs, gt = cfg3
+    idx.setBeam(0)
+    idx.setPrecision("fp32", 0)
+    idx.setQueryTimeParams({"algoType": "v1merge"})
+    ids, dists, counts, ms, wall = _timed_uneven(idx, q, ks, efs)
+    rec = _recall(ids, ks, gt)
+    print(f"cfg3 exact uneven: {len(q) / ms * 1e3:.0f} q/s (kernel {ms:.1f} ms, wall {wall * 1e3:.1f} ms),"
+          f" recall by k {rec}", flush=True)
+    oracle = O.OracleIndex(path)
+    for i in range(len(q))[SAMPLE]:
+        k, ef = int(ks[i]), int(efs[i])
+        oi, od, oc = oracle.search(q[i:i + 1], k, ef, "v1merge")
+        assert counts[i] == oc[0], i
+        np.testing.assert_array_equal(ids[i, :k], oi[0], err_msg=f"query {i} k={k} ef={ef}")
+        np.testing.assert_array_equal(dists[i, :k].view(np.uint32), od[0].view(np.uint32))
+
+
+@pytest.mark.parametrize("storage,rerank", [("fp32", 0), ("bf16", 2)])
+def test_cfg3_uneven_beam16(cfg3, storage, rerank):
+    path, idx, q, ks, efs, gt = cfg3
+    idx.setBeam(16)
+    idx.setPrecision(storage, rerank)
+    idx.setQueryTimeParams({"algoType": "v1merge"})
+    ids, dists, counts, ms, wall = _timed_uneven(idx, q, ks, efs)
+    rec = _recall(ids, ks, gt)
+    launch = idx.lastLaunch()
+    idx.setBeam(0)
+    idx.setPrecision("fp32", 0)
+    print(f"cfg3 {storage} beam16 rerank x{rerank} uneven: {len(q) / ms * 1e3:.0f} q/s "
+          f"(kernel {ms:.1f} ms, wall {wall * 1e3:.1f} ms), recall by k {rec}, {launch}", flush=True)
+    if storage != "fp32":
+        return
+    oracle = O.OracleIndex(path)
+    w = max(1, min(16, 512 // oracle.maxM0))
+    for i in range(len(q))[SAMPLE]:
+        k, ef = int(ks[i]), int(efs[i])
+        oi, od, oc = oracle.search_beam(q[i:i + 1], k, ef, w)
+        assert counts[i] == oc[0], i
+        np.testing.assert_array_equal(ids[i, :k], oi[0], err_msg=f"query {i} k={k} ef={ef}")
+        np.testing.assert_array_equal(dists[i, :k].view(np.uint32), od[0].view(np.uint32))
