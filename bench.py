@@ -159,6 +159,7 @@ FORMS = {
     "fp32-beam8": ("fp32", 0, 8),
     "fp32-beam16": ("fp32", 0, 16),
     "bf16-beam16-rerank": ("bf16", 2, 16),
+    "fp8-beam16-rerank": ("fp8", 2, 16),
 }
 
 

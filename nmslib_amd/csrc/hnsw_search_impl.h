@@ -382,8 +382,8 @@ __device__ __forceinline__ float vec_dist(const VecRegs<D, ST>& r, const float* 
 }
 
 template <int ST>
-struct WideNV {
-  static constexpr int value = ST == kF32 ? 2 : 4;
+struct WideNV {  // vectors in flight per lane pair: about 128 registers of payload
+  static constexpr int value = ST == kF32 ? 2 : (ST == kBF16 ? 4 : 8);
 };
 
 // cdist[s] = distance of cand[s] for s < ncand (fallback: a valid node id used
@@ -1027,10 +1027,13 @@ __device__ __forceinline__ void bitonic_sort(float (&k)[R], uint32_t (&id)[R], i
 // merged array is either old A[pa] (pa < 0: none) or sorted new item kx (kx >=
 // m: none), both unclaimed; its position goes to *first (nout when neither
 // survives): A[pa] moves up by #{j : L_j <= pa}, item j lands at L_j + j.
-template <int R, class F, class G>
+//
+// before_walk() runs once the records are built, before the rebuild (or in
+// place of it when nothing lands).
+template <int R, class F, class G, class H>
 __device__ __forceinline__ int sort_merge(uint2* A, int n_, int C_, uint2* X, int m_,
                                           uint4* meta, int lane, F&& after_sort, G&& lap,
-                                          const int2& nxt, int* first) {
+                                          const int2& nxt, int* first, H&& before_walk) {
   const int n = __builtin_amdgcn_readfirstlane(n_);
   const int C = __builtin_amdgcn_readfirstlane(C_);
   const int m = __builtin_amdgcn_readfirstlane(m_);
@@ -1091,6 +1094,7 @@ __device__ __forceinline__ int sort_merge(uint2* A, int n_, int C_, uint2* X, in
   }
   const int P0 = __builtin_amdgcn_readlane(L[0], 0);
   if (P0 >= nout) {
+    before_walk();
     lap(5);
     return P0;
   }
@@ -1125,6 +1129,7 @@ __device__ __forceinline__ int sort_merge(uint2* A, int n_, int C_, uint2* X, in
     base += tot;
   }
   WSYNC();
+  before_walk();
   lap(5);
   for (int c = chi; c >= clo; c -= 8) {
     uint4 md[8];
@@ -1254,7 +1259,7 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
       if (lane < naw) A[w.pfpos[lane]].y = ci | kUsedBit;
       else Xs[lane - naw].y = ci | kUsedBit;
     }
-    pf_n = min(W, tot);
+    pf_n = __builtin_amdgcn_readfirstlane(min(W, tot));
     // the claimed items of each side are a prefix of it
     const int ka = __popcll(__ballot(take && lane < naw));
     nxt = make_int2(ka < na ? w.pfpos[ka] : -1, pf_n - ka);
@@ -1271,6 +1276,14 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
     }
   };
   auto no_predict = [] {};
+  // this step's marks land before any later read of them, then the marks of
+  // the claimed picks' neighbours load (they arrive while the array is
+  // rebuilt and the next picks are taken)
+  auto load_marks = [&]() __attribute__((always_inline)) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+#pragma unroll
+    for (int r = 0; r < 8; ++r) vn[r] = (int)vis[tn[r] >= 0 ? tn[r] : 0];
+  };
   for (;;) {
     WSYNC();
     int nsel = 0;
@@ -1281,7 +1294,7 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
       const uint2 q = lane < pf_n ? w.pfq[lane] : make_uint2(0u, 0u);
       const bool surv = lane < pf_n && !kid_less(__uint_as_float(last.x), last.y & ~kUsedBit,
                                                  __uint_as_float(q.x), q.y);
-      nsel = __popcll(__ballot(surv));
+      nsel = __builtin_amdgcn_readfirstlane(__popcll(__ballot(surv)));
     } else {
       // the first W unexpanded items below min(n, ef), in position order
       const int lim = min(n, ef);
@@ -1396,18 +1409,24 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
       if (m <= 256 && claim) {
         auto pred = [&]() __attribute__((always_inline)) { predict(w.items, m); };
         int L0;
-        if (m <= 64) L0 = sort_merge<1>(A, n, C, w.items, m, w.meta, lane, pred, lap, nxt, &first);
-        else if (m <= 128) L0 = sort_merge<2>(A, n, C, w.items, m, w.meta, lane, pred, lap, nxt, &first);
-        else L0 = sort_merge<4>(A, n, C, w.items, m, w.meta, lane, pred, lap, nxt, &first);
+        if (m <= 64) L0 = sort_merge<1>(A, n, C, w.items, m, w.meta, lane, pred, lap, nxt, &first,
+                                     load_marks);
+        else if (m <= 128) L0 = sort_merge<2>(A, n, C, w.items, m, w.meta, lane, pred, lap, nxt, &first,
+                                     load_marks);
+        else L0 = sort_merge<4>(A, n, C, w.items, m, w.meta, lane, pred, lap, nxt, &first,
+                                     load_marks);
         n = min(C, n + m);
         ce = first;
       } else {
         for (int b = 0; b < m; b += 256) {
           const int cnt = min(256, m - b);
           int L0;
-          if (cnt <= 64) L0 = sort_merge<1>(A, n, C, w.items + b, cnt, w.meta, lane, no_predict, lap, nxt, nullptr);
-          else if (cnt <= 128) L0 = sort_merge<2>(A, n, C, w.items + b, cnt, w.meta, lane, no_predict, lap, nxt, nullptr);
-          else L0 = sort_merge<4>(A, n, C, w.items + b, cnt, w.meta, lane, no_predict, lap, nxt, nullptr);
+          if (cnt <= 64) L0 = sort_merge<1>(A, n, C, w.items + b, cnt, w.meta, lane, no_predict, lap, nxt,
+                                          nullptr, no_predict);
+          else if (cnt <= 128) L0 = sort_merge<2>(A, n, C, w.items + b, cnt, w.meta, lane, no_predict, lap, nxt,
+                                          nullptr, no_predict);
+          else L0 = sort_merge<4>(A, n, C, w.items + b, cnt, w.meta, lane, no_predict, lap, nxt,
+                                          nullptr, no_predict);
           n = min(C, n + cnt);
           if (L0 < ce) ce = L0;
           WSYNC();
@@ -1415,16 +1434,15 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
       }
     } else if (claim) {
       predict(w.items, 0);
+      load_marks();
       ce = nxt.x >= 0 ? nxt.x : n;
       lap(4);
     }
     WSYNC();
-    // this step's marks land before any later read of them, then the marks of
-    // the predicted picks' neighbours load while the next picks are found
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-#pragma unroll
-    for (int r = 0; r < 8; ++r) vn[r] = (int)vis[tn[r] >= 0 ? tn[r] : 0];
-    if (pf_n == 0) ce = first_unused(A, ce, n, lane);
+    if (pf_n == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      ce = first_unused(A, ce, n, lane);
+    }
     lap(7);
   }
   WSYNC();
