@@ -1141,10 +1141,12 @@ __device__ __forceinline__ int sort_merge(uint2* A, int n_, int C_, uint2* X, in
     for (int u = 0; u < 8; ++u) {
       const int cc = c - u;
       const int p = (cc << 6) + lane;
-      const uint64_t w = ((uint64_t)md[u].y << 32) | md[u].x;
-      const int bl = (int)md[u].z + __popcll(w & below_lane);
+      // marks below this lane: mbcnt counts the set bits of the mask below it
+      const int bl = (int)md[u].z + (int)__builtin_amdgcn_mbcnt_hi(
+                                        md[u].y, __builtin_amdgcn_mbcnt_lo(md[u].x, 0u));
+      const bool mine = ((lane < 32 ? md[u].x : md[u].y) >> (lane & 31)) & 1u;
       const bool ok = cc >= clo && p < nout;
-      const uint2* src = ((w >> lane) & 1ull) ? X + bl : A + (p - bl);
+      const uint2* src = mine ? X + bl : A + (p - bl);
       v[u] = *(ok ? src : A);
       pos[u] = ok ? p : -1;
     }
