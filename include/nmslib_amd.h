@@ -3,13 +3,14 @@
  *
  * One handle = one index object of python_bindings/nmslib.cc's
  * IndexWrapper<float> (nmslib.cc:68-376) restricted to method "hnsw" on the
- * dense float spaces "l2", "cosinesimil" and "negdotprod".  Every entry point
+ * dense float spaces "l2", "cosinesimil", "negdotprod", "l1" and "linf".  Every entry point
  * returns 0 on success or a negative status; the message of the last failure
  * on the calling thread is nmsamd_last_error().  No C++ exception crosses the
  * ABI and the library keeps no caller pointer past a call.  Results follow the
  * reference exactly: ids are the object ids stored in the index, distances
  * are what the optimized search reports (squared L2 for "l2",
- * 1 - cos for "cosinesimil", -dot for "negdotprod"), ascending.
+ * 1 - cos for "cosinesimil", -dot for "negdotprod", sum |q - x| for "l1",
+ * max |q - x| for "linf"), ascending.
  */
 #ifndef NMSLIB_AMD_H
 #define NMSLIB_AMD_H

@@ -28,7 +28,7 @@
 namespace nmsamd {
 
 // distance function codes of the optimized index (include/method/hnsw.h:50-58)
-enum DistFunc : int { kL2Sqr16 = 1, kL2Sqr = 2, kNormCos = 3, kNegDot = 4 };
+enum DistFunc : int { kL2Sqr16 = 1, kL2Sqr = 2, kNormCos = 3, kNegDot = 4, kL1 = 5, kLInf = 6 };
 // vector storage used while traversing
 enum Storage : int { kF32 = 0, kBF16 = 1, kFP8 = 2 };
 

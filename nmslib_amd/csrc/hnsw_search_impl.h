@@ -130,8 +130,81 @@ __device__ __forceinline__ float dist_pair_f32(const float* __restrict__ qs,
   return finish<FUNC>(s);
 }
 
+// L1NormSIMD / LInfNormSIMD (distcomp_lp.cc:78-247) by a lane pair: lane h
+// keeps SSE lanes 2h and 2h+1, fed by every group of 4 floats in order; the
+// four lanes are then summed left to right in float, the tail added in
+// double and the result returned as float (L1), or the largest |q - x| taken
+// (LInf, independent of order).  Both lanes return the same value.
+template <int FUNC>
+__device__ __forceinline__ float dist_pair_l1inf(const float* __restrict__ qs,
+                                                 const float* __restrict__ x, int h, int d) {
+  const int q4 = d >> 2;
+  float a0 = 0.f, a1 = 0.f;
+  for (int g = 0; g < q4; ++g) {
+    const float2 xv = *reinterpret_cast<const float2*>(x + 4 * g + 2 * h);
+    const float2 qv = *reinterpret_cast<const float2*>(qs + 4 * g + 2 * h);
+    const float t0 = fabsf(qv.x - xv.x), t1 = fabsf(qv.y - xv.y);
+    if constexpr (FUNC == kL1) {
+      a0 = a0 + t0;
+      a1 = a1 + t1;
+    } else {
+      a0 = a0 < t0 ? t0 : a0;
+      a1 = a1 < t1 ? t1 : a1;
+    }
+  }
+  const float p0 = __shfl_xor(a0, 1), p1 = __shfl_xor(a1, 1);
+  const float l0 = h ? p0 : a0, l1 = h ? p1 : a1;  // SSE lanes 0, 1
+  const float u0 = h ? a0 : p0, u1 = h ? a1 : p1;  // SSE lanes 2, 3
+  if constexpr (FUNC == kL1) {
+    float s = l0 + l1;
+    s = s + u0;
+    s = s + u1;
+    double r = s;
+    for (int c = 4 * q4; c < d; ++c) r += (double)fabsf(qs[c] - x[c]);
+    return (float)r;
+  } else {
+    const float m01 = l0 < l1 ? l1 : l0, m23 = u0 < u1 ? u1 : u0;
+    float r = m01 < m23 ? m23 : m01;
+    for (int c = 4 * q4; c < d; ++c) {
+      const float t = fabsf(qs[c] - x[c]);
+      r = r < t ? t : r;
+    }
+    return r;
+  }
+}
+
+// FP32 distance in the reference's order for the function
+template <int D, int FUNC>
+__device__ __forceinline__ float dist_f32(const float* __restrict__ qs, const float* __restrict__ x,
+                                          int h, int drt) {
+  if constexpr (FUNC == kL1 || FUNC == kLInf) return dist_pair_l1inf<FUNC>(qs, x, h, D > 0 ? D : drt);
+  else return dist_pair_f32<D, FUNC>(qs, x, h, drt);
+}
+
 __device__ __forceinline__ float bf16_to_f32(uint32_t bits16) {
   return __uint_as_float(bits16 << 16);
+}
+
+// one term of a reduced-precision distance, and the lane pair's combination
+template <int FUNC>
+__device__ __forceinline__ float lp_acc(float acc, float q, float x) {
+  if constexpr (FUNC == kL2Sqr16 || FUNC == kL2Sqr) {
+    const float t = q - x;
+    return __builtin_fmaf(t, t, acc);
+  } else if constexpr (FUNC == kL1) {
+    return acc + fabsf(q - x);
+  } else if constexpr (FUNC == kLInf) {
+    const float t = fabsf(q - x);
+    return acc < t ? t : acc;
+  } else {
+    return __builtin_fmaf(q, x, acc);
+  }
+}
+template <int FUNC>
+__device__ __forceinline__ float lp_pair(float acc) {
+  const float p = __shfl_xor(acc, 1);
+  if constexpr (FUNC == kLInf) return acc < p ? p : acc;
+  else return acc + p;
 }
 
 // Reduced-precision distance (BF16 or per-vector-scaled FP8 e4m3 storage,
@@ -176,14 +249,7 @@ __device__ __forceinline__ float dist_pair_lp(const float* __restrict__ qs, cons
     const float4 qb = *reinterpret_cast<const float4*>(qs + e0 + 4);
     const float qv[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      if constexpr (FUNC == kL2Sqr16 || FUNC == kL2Sqr) {
-        const float t = qv[j] - xv[j];
-        acc = __builtin_fmaf(t, t, acc);
-      } else {
-        acc = __builtin_fmaf(qv[j], xv[j], acc);
-      }
-    }
+    for (int j = 0; j < 8; ++j) acc = lp_acc<FUNC>(acc, qv[j], xv[j]);
   }
   for (int c = 16 * q16 + h; c < d; c += 2) {
     float xv;
@@ -193,22 +259,16 @@ __device__ __forceinline__ float dist_pair_lp(const float* __restrict__ qs, cons
       const auto p = __builtin_amdgcn_cvt_pk_f32_fp8((int)ix.vec8[node * ix.dstride + c], false);
       xv = p[0] * scale;
     }
-    if constexpr (FUNC == kL2Sqr16 || FUNC == kL2Sqr) {
-      const float t = qs[c] - xv;
-      acc = __builtin_fmaf(t, t, acc);
-    } else {
-      acc = __builtin_fmaf(qs[c], xv, acc);
-    }
+    acc = lp_acc<FUNC>(acc, qs[c], xv);
   }
-  acc = acc + __shfl_xor(acc, 1);
-  return finish<FUNC>(acc);
+  return finish<FUNC>(lp_pair<FUNC>(acc));
 }
 
 template <int D, int FUNC, int ST>
 __device__ __forceinline__ float dist_node(const float* __restrict__ qs, const DevIndex& ix,
                                            int64_t node, int h) {
   if constexpr (ST == kF32) {
-    return dist_pair_f32<D, FUNC>(qs, ix.vec + node * ix.dstride, h, ix.dim);
+    return dist_f32<D, FUNC>(qs, ix.vec + node * ix.dstride, h, ix.dim);
   } else {
     return dist_pair_lp<D, FUNC, ST>(qs, ix, node, h, ix.dim);
   }
@@ -351,14 +411,7 @@ __device__ __forceinline__ float vec_dist(const VecRegs<D, ST>& r, const float* 
       const float4 qb = *reinterpret_cast<const float4*>(qs + e0 + 4);
       const float qv[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        if constexpr (FUNC == kL2Sqr16 || FUNC == kL2Sqr) {
-          const float t = qv[j] - xv[j];
-          acc = __builtin_fmaf(t, t, acc);
-        } else {
-          acc = __builtin_fmaf(qv[j], xv[j], acc);
-        }
-      }
+      for (int j = 0; j < 8; ++j) acc = lp_acc<FUNC>(acc, qv[j], xv[j]);
     }
 #pragma unroll
     for (int t = 0; 2 * t + h < D % 16; ++t) {
@@ -369,15 +422,9 @@ __device__ __forceinline__ float vec_dist(const VecRegs<D, ST>& r, const float* 
       } else {
         xv = __builtin_amdgcn_cvt_pk_f32_fp8((int)r.t[t], false)[0] * scale;
       }
-      if constexpr (FUNC == kL2Sqr16 || FUNC == kL2Sqr) {
-        const float d = qs[c] - xv;
-        acc = __builtin_fmaf(d, d, acc);
-      } else {
-        acc = __builtin_fmaf(qs[c], xv, acc);
-      }
+      acc = lp_acc<FUNC>(acc, qs[c], xv);
     }
-    acc = acc + __shfl_xor(acc, 1);
-    return finish<FUNC>(acc);
+    return finish<FUNC>(lp_pair<FUNC>(acc));
   }
 }
 
@@ -392,7 +439,7 @@ template <int D, int FUNC, int ST, int WIDE>
 __device__ __forceinline__ void dist_cands(const float* __restrict__ qs, const DevIndex& ix,
                                            const int* cand, float* cdist, int ncand, int lane,
                                            int h, int fallback) {
-  if constexpr (WIDE && D > 0 && D % 4 == 0) {
+  if constexpr (WIDE && D > 0 && D % 4 == 0 && (ST != kF32 || FUNC <= kNegDot)) {
     constexpr int NV = WideNV<ST>::value;
     for (int b = 0; b < ncand; b += 32 * NV) {
       VecRegs<D, ST> r[NV];
@@ -623,7 +670,7 @@ __device__ __forceinline__ void finish_query(const SearchArgs& a, const WaveLds&
         const int s = b + (lane >> 1);
         const bool ok = s < R;
         const int id = ok ? (int)(A[s].y & ~kUsedBit) : 0;
-        const float dd = dist_pair_f32<D, FUNC>(w.qs, ix.vec + (int64_t)id * ix.dstride, h, ix.dim);
+        const float dd = dist_f32<D, FUNC>(w.qs, ix.vec + (int64_t)id * ix.dstride, h, ix.dim);
         if (ok && h == 0) A[s] = make_uint2(__float_as_uint(dd), (uint32_t)id);
       }
       WSYNC();

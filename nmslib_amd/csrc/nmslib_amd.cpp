@@ -193,11 +193,8 @@ void parse_and_upload(nmsamd_index* h, const std::string& path) {
     rd(in, h->maxM0);
     rd(in, h->func);
     rd(in, h->method);
-    if (h->func < kL2Sqr16 || h->func > 6)
+    if (h->func < kL2Sqr16 || h->func > kLInf)
       fail(NMSAMD_ERUNTIME, "Unknown distance function code: " + std::to_string(h->func));
-    if (h->func > kNegDot)
-      fail(NMSAMD_ERUNTIME, "distance function code " + std::to_string(h->func) +
-                                " (l1/linf) is not supported by the device path");
     if (h->off_data != 0 || h->off_l0 < 16 || (h->off_l0 - 16) % 4 != 0 ||
         h->elem != h->off_l0 + 4 * (h->maxM0 + 1))
       fail(NMSAMD_ERUNTIME, "inconsistent optimized-index header");
@@ -234,6 +231,10 @@ void parse_and_upload(nmsamd_index* h, const std::string& path) {
     fail(NMSAMD_ERUNTIME, "index distance function does not match space cosinesimil");
   if (h->expected_func == kNegDot && h->func != kNegDot)
     fail(NMSAMD_ERUNTIME, "index distance function does not match space negdotprod");
+  if (h->expected_func == kL1 && h->func != kL1)
+    fail(NMSAMD_ERUNTIME, "index distance function does not match space l1");
+  if (h->expected_func == kLInf && h->func != kLInf)
+    fail(NMSAMD_ERUNTIME, "index distance function does not match space linf");
 
   // device layout (hnsw_device.h)
   const int64_t n = h->n;
@@ -512,6 +513,8 @@ int nmsamd_init(const char* space, const char* method, nmsamd_index** out) {
     if (sp == "l2") h->expected_func = 1;
     else if (sp == "cosinesimil") h->expected_func = kNormCos;
     else if (sp == "negdotprod") h->expected_func = kNegDot;
+    else if (sp == "l1") h->expected_func = kL1;
+    else if (sp == "linf") h->expected_func = kLInf;
     else fail(NMSAMD_ERUNTIME, "It looks like the space " + sp +
                                    " is not defined for the distance type : FLOAT");
     if (me != "hnsw")
@@ -1083,6 +1086,8 @@ extern "C" int nmsamd_brute_force(nmsamd_index* h, const float* q, int64_t nq, i
     std::lock_guard<std::mutex> g(h->mu);
     if (!h->loaded) fail(NMSAMD_EINVAL, "Must call createIndex or loadIndex before this method");
     if (dim != h->dim) fail(NMSAMD_EINVAL, "query dimension does not match the index");
+    if (h->func == kL1 || h->func == kLInf)
+      fail(NMSAMD_EINVAL, "brute force on matrix cores covers l2, cosinesimil and negdotprod");
     if (h->dstride > 128) fail(NMSAMD_EINVAL, "brute force supports dimensions up to 128");
     if (k < 1 || k > 128) fail(NMSAMD_EINVAL, "brute force k must be in 1..128");
     if (nq <= 0) return;

@@ -10,6 +10,8 @@ KernelFn pick_kernel_bf16(int dim, int func, int mode) {
     case kL2Sqr: return kernel_of<D, kL2Sqr, kBF16, W>();          \
     case kNormCos: return kernel_of<D, kNormCos, kBF16, W>();      \
     case kNegDot: return kernel_of<D, kNegDot, kBF16, W>();        \
+    case kL1: return kernel_of<0, kL1, kBF16, W>();                 \
+    case kLInf: return kernel_of<0, kLInf, kBF16, W>();             \
     default: return nullptr;                                           \
   }
 #define NMS_DIMS(W)                \

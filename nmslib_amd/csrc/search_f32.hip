@@ -10,6 +10,8 @@ KernelFn pick_kernel_f32(int dim, int func, int mode) {
     case kL2Sqr: return kernel_of<D, kL2Sqr, kF32, W>();          \
     case kNormCos: return kernel_of<D, kNormCos, kF32, W>();      \
     case kNegDot: return kernel_of<D, kNegDot, kF32, W>();        \
+    case kL1: return kernel_of<0, kL1, kF32, W>();                 \
+    case kLInf: return kernel_of<0, kLInf, kF32, W>();             \
     default: return nullptr;                                           \
   }
 #define NMS_DIMS(W)                \

@@ -10,6 +10,8 @@ KernelFn pick_kernel_fp8(int dim, int func, int mode) {
     case kL2Sqr: return kernel_of<D, kL2Sqr, kFP8, W>();          \
     case kNormCos: return kernel_of<D, kNormCos, kFP8, W>();      \
     case kNegDot: return kernel_of<D, kNegDot, kFP8, W>();        \
+    case kL1: return kernel_of<0, kL1, kFP8, W>();                 \
+    case kLInf: return kernel_of<0, kLInf, kFP8, W>();             \
     default: return nullptr;                                           \
   }
 #define NMS_DIMS(W)                \

@@ -70,8 +70,31 @@ float ext_kernel(const float* a, const float* b, int64_t d) {
   return s;
 }
 
+// L1NormSIMD (distcomp_lp.cc:191-247): four SSE lanes fed by groups of 4
+// floats in order, the lanes summed left to right in float, the tail added in
+// double, the result returned as float.  |a - b| is exact in either order.
+float l1_kernel(const float* a, const float* b, int64_t d) {
+  float acc[4] = {0, 0, 0, 0};
+  int64_t c = 0;
+  for (; c + 4 <= d; c += 4)
+    for (int j = 0; j < 4; ++j) acc[j] = acc[j] + std::fabs(a[c + j] - b[c + j]);
+  double res = ((acc[0] + acc[1]) + acc[2]) + acc[3];
+  for (; c < d; ++c) res += std::fabs(a[c] - b[c]);
+  return (float)res;
+}
+
+// LInfNormSIMD (distcomp_lp.cc:78-139): the largest |a - b|; a maximum does
+// not depend on the order of its operands.
+float linf_kernel(const float* a, const float* b, int64_t d) {
+  float res = 0.0f;
+  for (int64_t c = 0; c < d; ++c) res = std::max(res, std::fabs(a[c] - b[c]));
+  return res;
+}
+
 float dist_fn(int func, const float* q, const float* x, int64_t d) {
   switch (func) {
+    case ORA_L1: return l1_kernel(q, x, d);
+    case ORA_LINF: return linf_kernel(q, x, d);
     case ORA_L2SQR16: return l2sqr16(q, x, d);
     case ORA_L2SQR: return ext_kernel<false>(q, x, d);
     case ORA_NORMCOS: {
@@ -126,7 +149,7 @@ Index* load(const char* path) {
   rd(in, ix->n); rd(in, ix->elem); rd(in, ix->off_l0); rd(in, ix->off_data);
   rd(in, ix->maxlevel); rd(in, ix->enter); rd(in, ix->maxM); rd(in, ix->maxM0);
   rd(in, ix->func); rd(in, ix->method);
-  if (ix->func < ORA_L2SQR16 || ix->func > ORA_NEGDOT)
+  if (ix->func < ORA_L2SQR16 || ix->func > ORA_LINF)
     throw std::runtime_error("Unknown distance function code: " + std::to_string(ix->func));
   ix->dim = ((int64_t)ix->off_l0 - 16) / 4;
   ix->block.resize((size_t)ix->n * ix->elem);
@@ -596,7 +619,7 @@ void ora_normalize(float* v, int64_t dim) { normalize(v, dim); }
 int ora_brute_force(const float* data, int64_t n, int64_t dim, const float* q, int64_t nq, int k,
                     int space, int num_threads, int64_t* ids, double* dists) {
   try {
-    if (space < 1 || space > 3) throw std::invalid_argument("bad space code");
+    if (space < 1 || space > 5) throw std::invalid_argument("bad space code");
     std::vector<double> xnorm(space == 2 ? n : 0);
     for (int64_t i = 0; i < (int64_t)xnorm.size(); ++i) {
       double s = 0;
@@ -612,7 +635,11 @@ int ora_brute_force(const float* data, int64_t n, int64_t dim, const float* q, i
       for (int64_t i = 0; i < n; ++i) {
         const float* x = data + i * dim;
         double s = 0;
-        if (space == 1) {
+        if (space == 4) {
+          for (int64_t j = 0; j < dim; ++j) s += std::fabs((double)qq[j] - x[j]);
+        } else if (space == 5) {
+          for (int64_t j = 0; j < dim; ++j) s = std::max(s, std::fabs((double)qq[j] - x[j]));
+        } else if (space == 1) {
           for (int64_t j = 0; j < dim; ++j) { double t = (double)qq[j] - x[j]; s += t * t; }
         } else {
           for (int64_t j = 0; j < dim; ++j) s += (double)qq[j] * x[j];

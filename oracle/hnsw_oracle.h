@@ -14,6 +14,7 @@
  *                           result heap include/knnqueue.h:55-74, src/knnquery.cc:67-75
  *   sorted array            include/sort_arr_bi.h:61-199
  *   ora_dist                include/method/hnsw_distfunc_opt_impl_inline.h:42-173 (AVX order),
+ *                           L1NormSIMD / LInfNormSIMD src/distcomp_lp.cc:78-247 (SSE order),
  *                           src/method/hnsw.cc:70-102
  *   ora_normalize           include/method/hnsw.h:486-497
  *   ora_brute_force         src/method/seqsearch.cc:143-177 ranked as
@@ -66,7 +67,7 @@ float ora_dist(int func, const float* q, const float* x, int64_t dim);
 void ora_normalize(float* v, int64_t dim);
 
 /* Exact kNN in FP64 over raw data; space: 1 l2 (squared), 2 cosine (1 - cos on raw
- * vectors), 3 negdot.  Ranked by (distance, index). */
+ * vectors), 3 negdot, 4 l1, 5 linf.  Ranked by (distance, index). */
 int ora_brute_force(const float* data, int64_t n, int64_t dim, const float* q, int64_t nq, int k,
                     int space, int num_threads, int64_t* ids, double* dists);
 

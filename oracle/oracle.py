@@ -15,9 +15,9 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
 
-L2SQR16, L2SQR, NORMCOS, NEGDOT = 1, 2, 3, 4
+L2SQR16, L2SQR, NORMCOS, NEGDOT, L1, LINF = 1, 2, 3, 4, 5, 6
 ALGO = {"hybrid": 0, "v1merge": 1, "old": 2}
-SPACE_BF = {"l2": 1, "cosinesimil": 2, "negdotprod": 3}
+SPACE_BF = {"l2": 1, "cosinesimil": 2, "negdotprod": 3, "l1": 4, "linf": 5}
 
 
 class Stats(C.Structure):

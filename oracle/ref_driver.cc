@@ -49,6 +49,8 @@ struct RefIndex {
 Space<float>* make_space(const char* name) {
   std::string s(name);
   if (s == "l2") return new SpaceLp<float>(2);
+  if (s == "l1") return new SpaceLp<float>(1);
+  if (s == "linf") return new SpaceLp<float>(-1);
   if (s == "cosinesimil") return new SpaceCosineSimilarity<float>();
   if (s == "negdotprod") return new SpaceNegativeScalarProduct<float>();
   throw std::runtime_error("ref driver: unsupported space " + s);

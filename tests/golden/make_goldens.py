@@ -42,6 +42,8 @@ FIXTURES = {
                    "M=32,efConstruction=400"),
     "gmm96_nd": ("negdotprod", "gmm", 2000, 96, 19, 200, 20, "M=24,efConstruction=200"),
     "tiny_l2": ("l2", "uniform", 60, 128, 21, 50, 22, "M=4,efConstruction=20"),
+    "u67_l1": ("l1", "uniform", 2000, 67, 23, 200, 24, "M=16,efConstruction=200"),
+    "u64_linf": ("linf", "uniform", 2000, 64, 25, 200, 26, "M=16,efConstruction=200"),
 }
 
 # (efSearch, k, algoType)

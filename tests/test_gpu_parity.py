@@ -37,7 +37,8 @@ def test_gpu_matches_reference_goldens(name):
                     f"{name} {t}")
 
 
-@pytest.mark.parametrize("name", ["u128_l2", "hn128_l2", "dec100_cos", "gmm96_nd"])
+@pytest.mark.parametrize("name", ["u128_l2", "hn128_l2", "dec100_cos", "gmm96_nd", "u67_l1",
+                                  "u64_linf"])
 @pytest.mark.parametrize("ef,k", [(24, 10), (48, 5), (200, 50), (300, 1)])
 def test_gpu_matches_oracle(name, ef, k):
     m = F.META[name]
@@ -193,7 +194,8 @@ def test_gpu_beam_mode_quality(name, beam):
     assert r_beam >= r_exact - 0.003, (r_beam, r_exact)
 
 
-@pytest.mark.parametrize("name", ["u128_l2", "hn128_l2", "dec100_cos", "gmm96_nd", "cfg1_l2"])
+@pytest.mark.parametrize("name", ["u128_l2", "hn128_l2", "dec100_cos", "gmm96_nd", "cfg1_l2",
+                                  "u67_l1", "u64_linf"])
 @pytest.mark.parametrize("beam", [1, 2, 5, 8, 16])
 def test_gpu_beam_mode_equals_model(name, beam):
     # the device's beam mode (bitonic sort + bitmap merge of each step's
@@ -235,7 +237,7 @@ def test_gpu_bf16_on_integer_data_is_exact():
 
 
 @pytest.mark.parametrize("storage", ["bf16", "fp8"])
-@pytest.mark.parametrize("name", ["u128_l2", "dec100_cos", "gmm96_nd"])
+@pytest.mark.parametrize("name", ["u128_l2", "dec100_cos", "gmm96_nd", "u67_l1", "u64_linf"])
 def test_gpu_reduced_precision_with_rerank(name, storage):
     m = F.META[name]
     idx = load(name, F.index_path(name), m["space"], 100)

@@ -38,11 +38,53 @@ def test_kat_distances():
     v = np.array(kat["vectors"], np.float32)
     l2 = np.array(kat["l2_sqrt"])
     nd = np.array(kat["negdotprod"])
+    l1 = np.array(kat["l1"])
+    linf = np.array(kat["linf"])
     for i in range(8):
         for j in range(8):
             d2 = O.dist(O.L2SQR, v[i], v[j])  # dim 5: the any-dim kernel
             assert abs(np.sqrt(d2) - l2[i, j]) <= 1e-5
             assert abs(O.dist(O.NEGDOT, v[i], v[j]) - nd[i, j]) <= 1e-5
+            assert abs(O.dist(O.L1, v[i], v[j]) - l1[i, j]) <= 1e-5
+            assert abs(O.dist(O.LINF, v[i], v[j]) - linf[i, j]) <= 1e-5
+
+
+@pytest.mark.parametrize("dim", [1, 3, 4, 5, 15, 16, 17, 64, 67, 100, 128])
+def test_l1_linf_against_fp64(dim):
+    rng = np.random.default_rng(100 + dim)
+    for _ in range(20):
+        a = rng.standard_normal(dim).astype(np.float32)
+        b = rng.standard_normal(dim).astype(np.float32)
+        exact = float(np.abs(a.astype(np.float64) - b).sum())
+        assert abs(O.dist(O.L1, a, b) - exact) <= 1e-5 * exact
+        # |a - b| of floats is exact, so the largest one is too
+        assert O.dist(O.LINF, a, b) == np.abs(a - b).max()
+
+
+def test_l1_summation_order_is_the_sse_one():
+    # L1NormSIMD (distcomp_lp.cc:191-247): four float lanes over groups of 4,
+    # lanes summed in float, the tail added in double; a plain float loop
+    # rounds differently on some of these inputs
+    rng = np.random.default_rng(7)
+    differs = 0
+    for _ in range(200):
+        dim = 67
+        a = (rng.standard_normal(dim) * 10 ** rng.uniform(-3, 3, dim)).astype(np.float32)
+        b = rng.standard_normal(dim).astype(np.float32)
+        t = np.abs(a - b)
+        lanes = np.zeros(4, np.float32)
+        for g in range(dim // 4):
+            lanes = (lanes + t[4 * g:4 * g + 4]).astype(np.float32)
+        s = np.float32(np.float32(np.float32(lanes[0] + lanes[1]) + lanes[2]) + lanes[3])
+        res = float(s)
+        for c in range(4 * (dim // 4), dim):
+            res += float(t[c])
+        assert O.dist(O.L1, a, b) == np.float32(res)
+        seq = np.float32(0)
+        for v in t:
+            seq = np.float32(seq + v)
+        differs += seq != np.float32(res)
+    assert differs > 0
 
 
 @pytest.mark.parametrize("dim", list(range(16, 257, 16)))
@@ -116,6 +158,10 @@ def test_oracle_brute_force_vs_reference_seq_search(name):
             qq = q[qi].astype(np.float64)
             if m["space"] == "l2":
                 d = np.sqrt(((qq - v) ** 2).sum())
+            elif m["space"] == "l1":
+                d = np.abs(qq - v).sum()
+            elif m["space"] == "linf":
+                d = np.abs(qq - v).max()
             elif m["space"] == "negdotprod":
                 d = -(qq @ v)
             else:
