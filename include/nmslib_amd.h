@@ -103,6 +103,13 @@ int nmsamd_set_precision(nmsamd_index* h, int storage, int rerank_mult);
  * ef >= 1000 does not apply); algoType=old still runs SearchOld. */
 int nmsamd_set_beam(nmsamd_index* h, int beam);
 
+/* Waves per query in beam mode: 1, 2 (a block of two waves per query: the
+ * second computes part of each step's distances and rebuilds half of the
+ * sorted array), or 0 (default) = two where the LDS footprint leaves room for
+ * fewer than two queries per SIMD.  Results do not depend on it.  No
+ * reference counterpart. */
+int nmsamd_set_beam_waves(nmsamd_index* h, int waves);
+
 /* Per-call counters of the last search (expansions, level-0 distance
  * evaluations, upper-level distance evaluations, adjacency ints read,
  * expansions that took the tie path) and the kernel time of the last search
@@ -126,9 +133,9 @@ int nmsamd_last_phase_us(const nmsamd_index* h, double* out8);
 
 /* Shape of the last search launch: kernel variant (0 = hnsw_search_kernel
  * with V1Merge / SearchOld, 1 = hnsw_search_kernel beam search, 2 =
- * hnsw_search_kernel_wide beam search), blocks, waves per block, LDS bytes
- * per wave, neighbour entries per step.  Diagnostic; no reference
- * counterpart. */
+ * hnsw_search_kernel_wide beam search, 3 = hnsw_search_kernel_pair beam search
+ * with two waves per query), blocks, queries per block, LDS bytes per query,
+ * neighbour entries per step.  Diagnostic; no reference counterpart. */
 int nmsamd_last_launch(const nmsamd_index* h, int64_t* out5);
 
 /* Exact k-NN over the loaded index's FP32 vectors on MFMA: the device

@@ -66,6 +66,7 @@ SIGNATURES = {
     "nmsamd_collect_stats": (C.c_int, [_P, C.c_int]),
     "nmsamd_last_query_us": (C.c_int, [_P, _P, _I64]),
     "nmsamd_set_beam": (C.c_int, [_P, C.c_int]),
+    "nmsamd_set_beam_waves": (C.c_int, [_P, C.c_int]),
     "nmsamd_brute_force": (C.c_int, [_P, _P, _I64, _I64, _I32, _P, _P, _P]),
     "nmsamd_last_phase_us": (C.c_int, [_P, _P]),
     "nmsamd_last_launch": (C.c_int, [_P, _P]),
@@ -185,6 +186,12 @@ class FloatIndex:
         (expansions per step, capped at 512 // maxM0)."""
         _check(load_library().nmsamd_set_beam(self._h, int(beam)))
 
+    def setBeamWaves(self, waves: int = 0):
+        """Waves per beam query: 1, 2 (the second wave computes part of each
+        step's distances and rebuilds half of the sorted array) or 0 = chosen
+        from the LDS footprint.  Results do not depend on it."""
+        _check(load_library().nmsamd_set_beam_waves(self._h, int(waves)))
+
     def collectStats(self, on: bool = True):
         _check(load_library().nmsamd_collect_stats(self._h, int(bool(on))))
 
@@ -211,9 +218,12 @@ class FloatIndex:
         out = np.zeros(5, np.int64)
         _check(load_library().nmsamd_last_launch(self._h, _ptr(out)))
         mode = int(out[0])
-        return {"kernel": "hnsw_search_kernel_wide" if mode == 2 else "hnsw_search_kernel",
-                "mode": ("v1merge/old", "beam", "beam (wide)")[mode], "blocks": int(out[1]),
-                "waves_per_block": int(out[2]), "lds_bytes_per_wave": int(out[3]),
+        kernel = {2: "hnsw_search_kernel_wide", 3: "hnsw_search_kernel_pair"}.get(
+            mode, "hnsw_search_kernel")
+        return {"kernel": kernel,
+                "mode": ("v1merge/old", "beam", "beam (wide)", "beam (two waves)")[mode],
+                "blocks": int(out[1]),
+                "queries_per_block": int(out[2]), "lds_bytes_per_query": int(out[3]),
                 "entries_per_step": int(out[4])}
 
     def lastQueryTimesUs(self, nq: int):

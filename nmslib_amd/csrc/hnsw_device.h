@@ -102,8 +102,10 @@ struct SearchArgs {
 enum AlgoMode : int { kAlgoHybrid = 0, kAlgoOld = 1, kAlgoV1Merge = 2 };
 
 // Kernel variants: every search mode, or beam mode only (lower register
-// pressure), the latter also with all of a step's vector loads in flight.
-enum KernelMode : int { kModeAll = 0, kModeBeam = 1, kModeBeamWide = 2 };
+// pressure), the latter also with all of a step's vector loads in flight, or
+// with two waves per query (a block of two waves: the second computes part of
+// each step's distances and rebuilds half of the sorted array).
+enum KernelMode : int { kModeAll = 0, kModeBeam = 1, kModeBeamWide = 2, kModeBeamPair = 3 };
 
 }  // namespace nmsamd
 #endif

@@ -467,6 +467,15 @@ __device__ __forceinline__ void dist_cands(const float* __restrict__ qs, const D
   }
 }
 
+// Candidates dist_cands() covers per round of loads (one candidate per lane
+// pair, or NV per pair on the wide path).
+template <int D, int FUNC, int ST, int WIDE>
+struct DistRound {
+  static constexpr int value =
+      (WIDE && D > 0 && D % 4 == 0 && (ST != kF32 || FUNC <= kNegDot)) ? 32 * WideNV<ST>::value
+                                                                        : 32;
+};
+
 // Per-wave LDS: the sorted array, the query, two scratch regions of 8 * ent
 // bytes (ent = neighbour entries per step, a multiple of 64, >= 128) whose
 // uses do not overlap in time within a step, and the merge's position bitmap:
@@ -491,6 +500,9 @@ struct WaveLds {
   uint2* pfa;      // [24] beam step: first unexpanded items of the array
   int* pfpos;      // [24] their positions
   uint2* pfq;      // [16] the next step's picks, claimed in advance
+  int* ctl;        // [24] two-wave queries: two alternating 8-word commands
+                   // from the first wave to the second, and the second
+                   // wave's count of admitted items (word 16)
 };
 
 __device__ __forceinline__ WaveLds carve(unsigned char* base, int cap, int dstride, int ent) {
@@ -512,10 +524,28 @@ __device__ __forceinline__ WaveLds carve(unsigned char* base, int cap, int dstri
   w.pfa = reinterpret_cast<uint2*>(w.meta + (cap / 64 + 2));
   w.pfq = w.pfa + 24;
   w.pfpos = reinterpret_cast<int*>(w.pfq + 16);
+  w.ctl = w.pfpos + 24;
   return w;
 }
 
 __device__ __forceinline__ float akey(const uint2* A, int i) { return __uint_as_float(A[i].x); }
+
+// The candidates cand[s0..s1) the beam step's snapshot admits (distance below
+// the last item's, or the array not yet full), as (distance, id) items in
+// candidate order at out[0..); returns their number.
+__device__ __forceinline__ int admit(const WaveLds& w, int s0, int s1, float topk, int nsnap, int ef,
+                                     int lane, uint64_t ltm, uint2* out) {
+  int m = 0;
+  for (int b = s0; b < s1; b += 64) {
+    const int s = b + lane;
+    const float dj = s < s1 ? w.cdist[s] : 0.f;
+    const bool keep = s < s1 && (dj < topk || nsnap < ef);
+    const uint64_t km = __ballot(keep);
+    if (keep) out[m + __popcll(km & ltm)] = make_uint2(__float_as_uint(dj), (uint32_t)w.cand[s]);
+    m += __popcll(km);
+  }
+  return m;
+}
 
 // number of keys of A[0..n) below v (lower) or not above v (upper)
 __device__ __forceinline__ int lower_count(const uint2* A, int n, float v) {
@@ -1056,6 +1086,73 @@ __device__ __forceinline__ void bitonic_sort(float (&k)[R], uint32_t (&id)[R], i
   }
 }
 
+// Two waves per query (kModeBeamPair): the first wave runs the search and
+// hands the second part of each step through commands in LDS.  Posting a
+// command is one workgroup barrier: the second wave waits at it, then reads
+// the command and carries it out; the first wave posts an empty command when
+// it needs the second one's results.  Commands alternate between two 8-word
+// slots, so the next one can be written while the last is still being read.
+// Only post() and walk_pair() pass barriers, and both waves derive the number
+// walk_pair() passes from the same command, so the barrier counts match.
+enum PairCmd : int { kCmdNop = 0, kCmdDist = 1, kCmdWalk = 2, kCmdExit = 3 };
+
+__device__ __forceinline__ void pair_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+struct PairLink {
+  int* ctl;            // the command slots (16 words) and the second wave's count
+  int seq;             // commands posted so far
+  const uint2* items;  // base of the admitted-items buffer
+  __device__ __forceinline__ void post(int lane, int cmd, int p1 = 0, int p2 = 0, int p3 = 0,
+                                       int p4 = 0, int p5 = 0, int p6 = 0) {
+    int4* c = reinterpret_cast<int4*>(ctl + 8 * (seq & 1));
+    if (lane == 0) {
+      c[0] = make_int4(cmd, p1, p2, p3);
+      c[1] = make_int4(p4, p5, p6, 0);
+    }
+    ++seq;
+    pair_sync();
+  }
+};
+
+// One wave's share of sort_merge's rebuild walk when two waves share the
+// query: rounds of 16 chunks from the top, the first wave (v = 0) rebuilding
+// the upper 8 and the second the lower 8.  An output position reads an old
+// position at most 256 (the new items of one pass) below it, i.e. at most 4
+// chunks lower: inside this round's chunks of either wave or the top of the
+// next round's.  So both waves finish a round's reads (barrier) before either
+// writes, and a round's writes never meet reads still to come.
+__device__ __forceinline__ void walk_pair(uint2* A, const uint2* X, const uint4* meta, int clo,
+                                          int chi, int nout, int lane, int v) {
+  const int nr = (chi - clo + 16) >> 4;
+  for (int t = 0; t < nr; ++t) {
+    const int c = chi - 16 * t - 8 * v;
+    uint2 val[8];
+    int pos[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int cc = c - u;
+      const bool okc = cc >= clo;
+      const uint4 md = meta[(okc ? cc : clo) - clo];
+      const int p = (cc << 6) + lane;
+      const int bl = (int)md.z + (int)__builtin_amdgcn_mbcnt_hi(
+                                     md.y, __builtin_amdgcn_mbcnt_lo(md.x, 0u));
+      const bool mine = ((lane < 32 ? md.x : md.y) >> (lane & 31)) & 1u;
+      const bool ok = okc && p < nout;
+      const uint2* src = mine ? X + bl : A + (p - bl);
+      val[u] = *(ok ? src : A);
+      pos[u] = ok ? p : -1;
+    }
+    pair_sync();
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (pos[u] >= 0) A[pos[u]] = val[u];
+  }
+}
+
 // Sorts the m (<= 64 R) items of X[0..m) by (key, id), writes them back in
 // order and merges them into A[0..n), truncating at C.  Returns the position
 // the smallest new item takes (>= C: none is kept).
@@ -1077,10 +1174,11 @@ __device__ __forceinline__ void bitonic_sort(float (&k)[R], uint32_t (&id)[R], i
 //
 // before_walk() runs once the records are built, before the rebuild (or in
 // place of it when nothing lands).
-template <int R, class F, class G, class H>
+template <int R, bool PAIR, class F, class G, class H>
 __device__ __forceinline__ int sort_merge(uint2* A, int n_, int C_, uint2* X, int m_,
                                           uint4* meta, int lane, F&& after_sort, G&& lap,
-                                          const int2& nxt, int* first, H&& before_walk) {
+                                          const int2& nxt, int* first, H&& before_walk,
+                                          PairLink& link) {
   const int n = __builtin_amdgcn_readfirstlane(n_);
   const int C = __builtin_amdgcn_readfirstlane(C_);
   const int m = __builtin_amdgcn_readfirstlane(m_);
@@ -1178,6 +1276,13 @@ __device__ __forceinline__ int sort_merge(uint2* A, int n_, int C_, uint2* X, in
   WSYNC();
   before_walk();
   lap(5);
+  if constexpr (PAIR) {
+    link.post(lane, kCmdWalk, clo, chi, nout, (int)(X - link.items));
+    walk_pair(A, X, meta, clo, chi, nout, lane, 0);
+    link.post(lane, kCmdNop);  // the second wave's chunks are written
+    lap(6);
+    return P0;
+  }
   for (int c = chi; c >= clo; c -= 8) {
     uint4 md[8];
 #pragma unroll
@@ -1205,9 +1310,10 @@ __device__ __forceinline__ int sort_merge(uint2* A, int n_, int C_, uint2* X, in
   return P0;
 }
 
-template <int D, int FUNC, int ST, int WIDE>
+template <int D, int FUNC, int ST, int WIDE, bool PAIR>
 __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, uint8_t ep,
-                           int qi, int k, int ef, int lane, int h, uint64_t ltm) {
+                           int qi, int k, int ef, int lane, int h, uint64_t ltm,
+                           PairLink& link) {
   const DevIndex& ix = a.ix;
   const int C = max(ef, k);
   const int E = a.ent;  // entries per step: W * maxM0 <= E <= 512
@@ -1404,20 +1510,36 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
       fresh[r] = valid && vb[r] != (int)ep;
     }
     nl += nsel;
-    // a neighbour shared by two picks is taken once (LDS hash, linear probing)
+    // a neighbour shared by two picks is taken once (LDS hash, linear
+    // probing).  Every row's compare-and-swap goes out in one round without
+    // branching (a row with nothing to insert compares its slot against a
+    // value no slot holds, which changes nothing); rows that met another id probe
+    // the next slot in the next round.  Copies of one id follow the same probe
+    // sequence, so exactly one of them inserts and the others see it.
+    {
+      uint32_t slot[8];
+      bool pend[8];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      if (r < erows && fresh[r]) {
-        uint32_t slot = (uint32_t)(((uint64_t)((uint32_t)t[r] * 2654435761u) * hslots) >> 32);
-        for (;;) {
-          const uint32_t old = atomicCAS(&w.htab[slot], ~0u, (uint32_t)t[r]);
-          if (old == ~0u) break;
-          if (old == (uint32_t)t[r]) {
-            fresh[r] = false;
-            break;
-          }
-          if (++slot == hslots) slot = 0;
+      for (int r = 0; r < 8; ++r) {
+        pend[r] = r < erows && fresh[r];
+        slot[r] = (uint32_t)(((uint64_t)((uint32_t)t[r] * 2654435761u) * hslots) >> 32);
+      }
+      for (;;) {
+        uint32_t old[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+          old[r] = atomicCAS(&w.htab[slot[r]], pend[r] ? ~0u : 0xfffffffeu, (uint32_t)t[r]);
+        bool more = false;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          const bool ins = pend[r] && old[r] == ~0u;
+          const bool dup = pend[r] && old[r] == (uint32_t)t[r];
+          if (dup) fresh[r] = false;
+          pend[r] = pend[r] && !ins && !dup;
+          if (pend[r]) slot[r] = slot[r] + 1 == hslots ? 0u : slot[r] + 1;
+          more |= pend[r];
         }
+        if (__ballot(more) == 0) break;
       }
     }
 #pragma unroll
@@ -1435,18 +1557,39 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
     nd0 += ncand;
     WSYNC();
     lap(1);
-    dist_cands<D, FUNC, ST, WIDE>(w.qs, ix, w.cand, w.cdist, ncand, lane, h, cur);
+    // with two waves the second takes the candidates from h0 on, on the wide
+    // path (several vectors in flight per lane pair; this wave keeps the
+    // registers of the merge instead), both waves taking the same number of
+    // rounds of loads; for FP32 vectors the second wave takes them all (this
+    // wave's one-vector rounds are the slower ones there)
+    int h0 = ncand;
+    if constexpr (PAIR) {
+      constexpr int P0 = ST == kF32 ? 0 : DistRound<D, FUNC, ST, WIDE>::value;
+      constexpr int P1 = DistRound<D, FUNC, ST, true>::value;
+      const int r = (ncand + P0 + P1 - 1) / (P0 + P1);
+      h0 = max(0, ncand - P1 * r);
+      if (h0 < ncand) link.post(lane, kCmdDist, ncand, h0, __float_as_int(topk), nsnap, ef, cur);
+    }
+    dist_cands<D, FUNC, ST, WIDE>(w.qs, ix, w.cand, w.cdist, h0, lane, h, cur);
     WSYNC();
     lap(2);
     // keep what the snapshot admits; order does not matter (sorted next)
-    int m = 0;
-    for (int s0 = 0; s0 < ncand; s0 += 64) {
-      const int s = s0 + lane;
-      const float dj = s < ncand ? w.cdist[s] : 0.f;
-      const bool keep = s < ncand && (dj < topk || nsnap < ef);
-      const uint64_t km = __ballot(keep);
-      if (keep) w.items[m + __popcll(km & ltm)] = make_uint2(__float_as_uint(dj), (uint32_t)w.cand[s]);
-      m += __popcll(km);
+    int m = admit(w, 0, h0, topk, nsnap, ef, lane, ltm, w.items);
+    if constexpr (PAIR) {
+      if (h0 < ncand) {
+        link.post(lane, kCmdNop);  // the second wave's items are in
+        const int m1 = __builtin_amdgcn_readfirstlane(w.ctl[16]);
+        // they sit at items[h0..): move them down behind ours (m <= h0, so no
+        // chunk overwrites an item a later chunk still reads)
+        if (m < h0) {
+          for (int b = 0; b < m1; b += 64) {
+            const int i = b + lane;
+            const uint2 v = i < m1 ? w.items[h0 + i] : make_uint2(0u, 0u);
+            if (i < m1) w.items[m + i] = v;
+          }
+        }
+        m += m1;
+      }
     }
     WSYNC();
     lap(3);
@@ -1458,24 +1601,24 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
       if (m <= 256 && claim) {
         auto pred = [&]() __attribute__((always_inline)) { predict(w.items, m); };
         int L0;
-        if (m <= 64) L0 = sort_merge<1>(A, n, C, w.items, m, w.meta, lane, pred, lap, nxt, &first,
-                                     load_marks);
-        else if (m <= 128) L0 = sort_merge<2>(A, n, C, w.items, m, w.meta, lane, pred, lap, nxt, &first,
-                                     load_marks);
-        else L0 = sort_merge<4>(A, n, C, w.items, m, w.meta, lane, pred, lap, nxt, &first,
-                                     load_marks);
+        if (m <= 64) L0 = sort_merge<1, PAIR>(A, n, C, w.items, m, w.meta, lane, pred, lap, nxt, &first,
+                                     load_marks, link);
+        else if (m <= 128) L0 = sort_merge<2, PAIR>(A, n, C, w.items, m, w.meta, lane, pred, lap, nxt, &first,
+                                     load_marks, link);
+        else L0 = sort_merge<4, PAIR>(A, n, C, w.items, m, w.meta, lane, pred, lap, nxt, &first,
+                                     load_marks, link);
         n = min(C, n + m);
         ce = first;
       } else {
         for (int b = 0; b < m; b += 256) {
           const int cnt = min(256, m - b);
           int L0;
-          if (cnt <= 64) L0 = sort_merge<1>(A, n, C, w.items + b, cnt, w.meta, lane, no_predict, lap, nxt,
-                                          nullptr, no_predict);
-          else if (cnt <= 128) L0 = sort_merge<2>(A, n, C, w.items + b, cnt, w.meta, lane, no_predict, lap, nxt,
-                                          nullptr, no_predict);
-          else L0 = sort_merge<4>(A, n, C, w.items + b, cnt, w.meta, lane, no_predict, lap, nxt,
-                                          nullptr, no_predict);
+          if (cnt <= 64) L0 = sort_merge<1, PAIR>(A, n, C, w.items + b, cnt, w.meta, lane, no_predict, lap, nxt,
+                                          nullptr, no_predict, link);
+          else if (cnt <= 128) L0 = sort_merge<2, PAIR>(A, n, C, w.items + b, cnt, w.meta, lane, no_predict, lap, nxt,
+                                          nullptr, no_predict, link);
+          else L0 = sort_merge<4, PAIR>(A, n, C, w.items + b, cnt, w.meta, lane, no_predict, lap, nxt,
+                                          nullptr, no_predict, link);
           n = min(C, n + cnt);
           if (L0 < ce) ce = L0;
           WSYNC();
@@ -1499,6 +1642,43 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
     for (int p = 0; p < 8; ++p) atomicAdd(&a.stats->phase_ticks[p], ph[p]);
   const QueryCounters c{nexp, nd0, ndu, nl, 0};
   finish_query<D, FUNC, ST>(a, w, n, qi, k, lane, h, c);
+}
+
+// The second wave of a two-wave query (kModeBeamPair): carries out the
+// first wave's commands until it posts kCmdExit.
+//   kCmdDist (ncand, h0, topk, nsnap, ef, fallback id): distances of
+//     cand[h0..ncand) and the admitted ones as items at items[h0..); their
+//     number goes to ctl[16];
+//   kCmdWalk (clo, chi, nout, offset of the new items in items): the lower
+//     half of each round of the rebuild walk.
+template <int D, int FUNC, int ST>
+__device__ __forceinline__ void beam_helper(const SearchArgs& a, const WaveLds& w, int lane, int h,
+                                            uint64_t ltm) {
+  int seq = 0;
+  for (;;) {
+    pair_sync();
+    const int4* c = reinterpret_cast<const int4*>(w.ctl + 8 * (seq & 1));
+    ++seq;
+    const int4 c0 = c[0], c1 = c[1];
+    const int cmd = __builtin_amdgcn_readfirstlane(c0.x);
+    if (cmd == kCmdExit) break;
+    if (cmd == kCmdDist) {
+      const int ncand = __builtin_amdgcn_readfirstlane(c0.y);
+      const int h0 = __builtin_amdgcn_readfirstlane(c0.z);
+      const float topk = __int_as_float(__builtin_amdgcn_readfirstlane(c0.w));
+      const int nsnap = __builtin_amdgcn_readfirstlane(c1.x);
+      const int ef = __builtin_amdgcn_readfirstlane(c1.y);
+      const int fb = __builtin_amdgcn_readfirstlane(c1.z);
+      dist_cands<D, FUNC, ST, true>(w.qs, a.ix, w.cand + h0, w.cdist + h0, ncand - h0, lane, h, fb);
+      WSYNC();
+      const int m1 = admit(w, h0, ncand, topk, nsnap, ef, lane, ltm, w.items + h0);
+      if (lane == 0) w.ctl[16] = m1;
+    } else if (cmd == kCmdWalk) {
+      walk_pair(w.A, w.items + __builtin_amdgcn_readfirstlane(c1.x), w.meta,
+                __builtin_amdgcn_readfirstlane(c0.y), __builtin_amdgcn_readfirstlane(c0.z),
+                __builtin_amdgcn_readfirstlane(c0.w), lane, 1);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1652,8 +1832,9 @@ __device__ __forceinline__ void old_query(const SearchArgs& a, const WaveLds& w,
 // algoType / ef).  kModeBeam / kModeBeamWide run the beam search for every
 // query of the batch, each kernel carrying only its own loop's register
 // state; kModeBeamWide keeps a round's vector loads in flight together (one
-// wave per SIMD) and is used when the LDS footprint already limits residency
-// to about a wave per SIMD.
+// wave per SIMD).  kModeBeamPair gives each query a block of two waves: the
+// first runs the search, the second computes the later rounds of each step's
+// distances and rebuilds half of the sorted array (beam_helper).
 template <int D, int FUNC, int ST, int MODE>
 __device__ __forceinline__ void search_body(const SearchArgs& a, int lds_wave_bytes) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -1661,8 +1842,18 @@ __device__ __forceinline__ void search_body(const SearchArgs& a, int lds_wave_by
   const int lane = threadIdx.x & 63;
   const int h = lane & 1;
   const uint64_t ltm = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  const WaveLds w = carve(smem + wave * lds_wave_bytes, a.cap, a.ix.dstride, a.ent);
-  const int64_t slot = a.slot_base + (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+  // two-wave queries: one query per block, both waves on the same LDS
+  constexpr bool PAIR = MODE == kModeBeamPair;
+  const WaveLds w = carve(smem + (PAIR ? 0 : wave * lds_wave_bytes), a.cap, a.ix.dstride, a.ent);
+  if constexpr (PAIR) {
+    if (wave == 1) {
+      beam_helper<D, FUNC, ST>(a, w, lane, h, ltm);
+      return;
+    }
+  }
+  PairLink link{w.ctl, 0, w.items};
+  const int64_t slot = a.slot_base + (PAIR ? (int64_t)blockIdx.x
+                                           : (int64_t)blockIdx.x * (blockDim.x >> 6) + wave);
   uint8_t* vis = a.ix.visited + slot * a.ix.vstride;
   const int dim = a.ix.dim;
   const int64_t nq = a.nq_dev ? *a.nq_dev : a.nq;
@@ -1724,12 +1915,14 @@ __device__ __forceinline__ void search_body(const SearchArgs& a, int lds_wave_by
       else
         v1merge_query<D, FUNC, ST>(a, w, vis, ep, qi, k, ef, lane, h, ltm);
     } else {
-      beam_query<D, FUNC, ST, MODE == kModeBeamWide>(a, w, vis, ep, qi, k, ef, lane, h, ltm);
+      beam_query<D, FUNC, ST, MODE == kModeBeamWide, PAIR>(a, w, vis, ep, qi, k, ef, lane, h, ltm,
+                                                          link);
     }
     WSYNC();
     if (a.q_ticks && lane == 0)
       a.q_ticks[qi] = (uint32_t)(__builtin_amdgcn_s_memrealtime() - t_start);
   }
+  if constexpr (PAIR) link.post(lane, kCmdExit);
 }
 
 // kModeAll / kModeBeam: register budget of two waves per SIMD
@@ -1746,11 +1939,19 @@ __global__ void __launch_bounds__(kWave* kWavesPerBlock)
   search_body<D, FUNC, ST, kModeBeamWide>(a, lds_wave_bytes);
 }
 
+// kModeBeamPair: blocks of two waves sharing one query, two waves per SIMD
+template <int D, int FUNC, int ST>
+__global__ void __launch_bounds__(2 * kWave) __attribute__((amdgpu_waves_per_eu(2)))
+    hnsw_search_kernel_pair(SearchArgs a, int lds_wave_bytes) {
+  search_body<D, FUNC, ST, kModeBeamPair>(a, lds_wave_bytes);
+}
+
 using KernelFn = void (*)(SearchArgs, int);
 
 template <int D, int FUNC, int ST, int MODE>
 KernelFn kernel_of() {
   if constexpr (MODE == kModeBeamWide) return hnsw_search_kernel_wide<D, FUNC, ST>;
+  else if constexpr (MODE == kModeBeamPair) return hnsw_search_kernel_pair<D, FUNC, ST>;
   else return hnsw_search_kernel<D, FUNC, ST, MODE>;
 }
 

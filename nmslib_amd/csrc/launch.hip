@@ -21,13 +21,16 @@ hipError_t launch_search(const SearchArgs& a, int blocks, int waves_per_block, i
   else fn = pick_kernel_fp8(a.ix.dim, a.ix.func, mode);
   if (!fn) return hipErrorInvalidValue;
   if (waves_per_block < 1 || waves_per_block > kWavesPerBlock) return hipErrorInvalidValue;
+  // kModeBeamPair: one query per block, served by two waves on one LDS area
+  const bool pair = mode == kModeBeamPair;
+  if (pair && waves_per_block != 1) return hipErrorInvalidValue;
   const size_t lds = (size_t)lds_wave_bytes * waves_per_block;
   if (lds > 64 * 1024) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(fn, dim3(blocks), dim3(kWave * waves_per_block), lds, stream, a,
+  hipLaunchKernelGGL(fn, dim3(blocks), dim3(kWave * (pair ? 2 : waves_per_block)), lds, stream, a,
                      lds_wave_bytes);
   return hipGetLastError();
 }

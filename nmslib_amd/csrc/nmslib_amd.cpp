@@ -103,6 +103,7 @@ struct nmsamd_index {
   int storage = kF32;
   int rerank_mult = 0;
   int beam = 0;  // 0: exact V1Merge; > 0: beam mode width
+  int beam_waves = 0;  // waves per beam query: 0 = by LDS footprint, 1 or 2
   bool collect = false;
   // file header (hnsw.cc:779-788)
   bool loaded = false;
@@ -364,11 +365,11 @@ int step_entries(const nmsamd_index* h) {
 }
 
 // sorted array + query + two scratch regions of 8 * ent bytes; beam mode
-// adds the merge's 16-byte record per 64 positions and the claimed next picks
-// (416 bytes)
+// adds the merge's 16-byte record per 64 positions, the claimed next picks
+// (416 bytes) and the two-wave command words (96 bytes)
 int lds_wave_bytes(int cap, int dstride, int ent, bool beam) {
   int b = ((cap + 1) & ~1) * 8 + dstride * 4 + 16 * ent;
-  if (beam) b += (cap / 64 + 2) * 16 + 416;
+  if (beam) b += (cap / 64 + 2) * 16 + 416 + 96;
   return (b + 15) / 16 * 16;
 }
 
@@ -395,11 +396,21 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
     fail(NMSAMD_EINVAL, "efSearch / k too large: the search array of " + std::to_string(cap) +
                             " items exceeds the LDS of one wave");
   ensure_lowp(h);
-  // waves per block and resident blocks per CU from the LDS footprint; the
-  // grid is persistent: as many resident waves as the device holds, at most
-  // one per query
-  const int wpb = (int)std::max<size_t>(1, std::min<size_t>(kWavesPerBlock, kLdsPerCu / lds));
-  const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, kLdsPerCu / ((size_t)lds * wpb)));
+  // queries per block and resident blocks per CU from the LDS footprint; the
+  // grid is persistent: as many resident queries as the device holds, at most
+  // one per query.  Beam batches take the beam-only kernels: where the LDS
+  // footprint leaves room for fewer than two queries per SIMD, each query gets
+  // two waves (kModeBeamPair, one query per block, at most two waves per
+  // SIMD), else one.
+  // With one wave per query, the wide beam kernel (registers for one wave
+  // per SIMD) is taken where the footprint limits residency to that.
+  const size_t fit = kLdsPerCu / lds;
+  const bool pair = beam && (h->beam_waves == 2 || (h->beam_waves == 0 && fit < 8));
+  const int wpb = pair ? 1 : (int)std::max<size_t>(1, std::min<size_t>(kWavesPerBlock, fit));
+  const int per_cu = (int)std::max<size_t>(
+      1, std::min<size_t>(pair ? 4 : 8, kLdsPerCu / ((size_t)lds * wpb)));
+  const int mode = !beam ? kModeAll
+                         : (pair ? kModeBeamPair : (wpb * per_cu <= 4 ? kModeBeamWide : kModeBeam));
   int64_t max_blocks = (int64_t)h->cu_count * per_cu;
   // visited marks take n bytes per slot: keep them under ~16 GiB
   const int64_t vis_budget = (int64_t)16 << 30;
@@ -451,11 +462,6 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   a.old_heap_id = h->d_oldi.p;
   a.old_heap_cap = heap_cap;
   hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
-  // beam batches take the beam-only kernels; the wide one (registers for one
-  // wave per SIMD) keeps a round's gathers in flight together and is chosen
-  // when the LDS footprint already limits residency to that, the other one
-  // holds two waves per SIMD
-  const int mode = beam ? (wpb * per_cu <= 4 ? kModeBeamWide : kModeBeam) : kModeAll;
   hip_check(launch_search(a, blocks, wpb, lds, mode, s), "search kernel launch");
   hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
   h->last_launch[0] = mode;
@@ -767,6 +773,14 @@ int nmsamd_set_beam(nmsamd_index* h, int beam) {
     if (!h) fail(NMSAMD_EINVAL, "null handle");
     if (beam < 0 || beam > 16) fail(NMSAMD_EINVAL, "beam must be 0 (exact V1Merge) or 1..16");
     h->beam = beam;
+  });
+}
+
+int nmsamd_set_beam_waves(nmsamd_index* h, int waves) {
+  return guarded([&] {
+    if (!h) fail(NMSAMD_EINVAL, "null handle");
+    if (waves < 0 || waves > 2) fail(NMSAMD_EINVAL, "beam waves must be 0 (by LDS footprint), 1 or 2");
+    h->beam_waves = waves;
   });
 }
 

@@ -21,7 +21,9 @@ KernelFn pick_kernel_bf16(int dim, int func, int mode) {
     case 128: NMS_CASE(128, W)     \
     default: NMS_CASE(0, W)        \
   }
-  if (mode == kModeBeamWide) {
+  if (mode == kModeBeamPair) {
+    NMS_DIMS(kModeBeamPair)
+  } else if (mode == kModeBeamWide) {
     NMS_DIMS(kModeBeamWide)
   } else if (mode == kModeBeam) {
     NMS_DIMS(kModeBeam)

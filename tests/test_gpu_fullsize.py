@@ -61,16 +61,20 @@ def test_fullsize_exact_search_equals_oracle(big, ef):
 
 
 @pytest.mark.parametrize("beam", [4, 16])
-def test_fullsize_beam_search_equals_model(big, beam):
+@pytest.mark.parametrize("waves", [1, 2])
+def test_fullsize_beam_search_equals_model(big, beam, waves):
     # at the benchmark's efSearch the array spans 74 chunks of 64 and a step
     # merges up to a few hundred items: the device equals the CPU model of the
-    # beam search row for row
+    # beam search row for row, with one wave per query and with two
     path, idx, q, _, _ = big
     idx.setBeam(beam)
+    idx.setBeamWaves(waves)
     idx.setPrecision("fp32", 0)
     idx.setQueryTimeParams({"efSearch": 4736, "algoType": "v1merge"})
     ids, dists, counts = idx.knnQueryBatchArrays(q[:48], 10)
+    print(f"beam {beam} waves {waves}: {idx.lastLaunch()}", flush=True)
     idx.setBeam(0)
+    idx.setBeamWaves(0)
     oi, od, oc = O.OracleIndex(path).search_beam(q[:48], 10, 4736, beam, num_threads=8)
     np.testing.assert_array_equal(ids, oi)
     np.testing.assert_array_equal(dists.view(np.uint32), od.view(np.uint32))

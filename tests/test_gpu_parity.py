@@ -197,30 +197,35 @@ def test_gpu_beam_mode_quality(name, beam):
 @pytest.mark.parametrize("name", ["u128_l2", "hn128_l2", "dec100_cos", "gmm96_nd", "cfg1_l2",
                                   "u67_l1", "u64_linf"])
 @pytest.mark.parametrize("beam", [1, 2, 5, 8, 16])
-def test_gpu_beam_mode_equals_model(name, beam):
+@pytest.mark.parametrize("waves", [1, 2])
+def test_gpu_beam_mode_equals_model(name, beam, waves):
     # the device's beam mode (bitonic sort + bitmap merge of each step's
     # admitted items) against the CPU model of the same algorithm: ids,
     # distances and work counters identical; array sizes from one to a few
-    # chunks, k > ef, and steps of up to 512 entries
+    # chunks, k > ef, and steps of up to 512 entries; one wave per query, and
+    # two (the second wave computing part of the distances and rebuilding
+    # half of the array)
     m = F.META[name]
     idx = load(name, F.index_path(name), m["space"])
     oracle = O.OracleIndex(F.index_path(name))
     w = max(1, min(beam, 512 // oracle.maxM0))
     q = F.results(name)["queries"]
     idx.setBeam(beam)
+    idx.setBeamWaves(waves)
     idx.collectStats(True)
     for k, ef in [(10, 64), (100, 32), (10, 300), (1, 700)]:
         idx.setQueryTimeParams({"efSearch": ef, "algoType": "v1merge"})
         ids, dists, counts = idx.knnQueryBatchArrays(q, k)
         st = idx.lastStats()
         rids, rdists, rcounts, rst = oracle.search_beam(q, k, ef, w, num_threads=8, with_stats=True)
-        msg = f"{name} beam={beam} k={k} ef={ef}"
+        msg = f"{name} beam={beam} waves={waves} k={k} ef={ef} {idx.lastLaunch()['kernel']}"
         print(msg, flush=True)
         np.testing.assert_array_equal(counts, rcounts, err_msg=msg)
         np.testing.assert_array_equal(ids, rids, err_msg=msg)
         np.testing.assert_array_equal(dists.view(np.uint32), rdists.view(np.uint32), err_msg=msg)
         for key in ("expansions", "dist_level0", "dist_upper", "list_ints_read"):
             assert st[key] == rst[key], (msg, key, st[key], rst[key])
+    idx.setBeamWaves(0)
 
 
 def test_gpu_bf16_on_integer_data_is_exact():
