@@ -1094,7 +1094,7 @@ __device__ __forceinline__ void bitonic_sort(float (&k)[R], uint32_t (&id)[R], i
 // slots, so the next one can be written while the last is still being read.
 // Only post() and walk_pair() pass barriers, and both waves derive the number
 // walk_pair() passes from the same command, so the barrier counts match.
-enum PairCmd : int { kCmdNop = 0, kCmdDist = 1, kCmdWalk = 2, kCmdExit = 3 };
+enum PairCmd : int { kCmdNop = 0, kCmdDist = 1, kCmdWalk = 2, kCmdRank = 3, kCmdExit = 4 };
 
 __device__ __forceinline__ void pair_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -1106,6 +1106,7 @@ struct PairLink {
   int* ctl;            // the command slots (16 words) and the second wave's count
   int seq;             // commands posted so far
   const uint2* items;  // base of the admitted-items buffer
+  int* lbuf;           // the second wave's ranks of the new items (kCmdRank)
   __device__ __forceinline__ void post(int lane, int cmd, int p1 = 0, int p2 = 0, int p3 = 0,
                                        int p4 = 0, int p5 = 0, int p6 = 0) {
     int4* c = reinterpret_cast<int4*>(ctl + 8 * (seq & 1));
@@ -1151,6 +1152,63 @@ __device__ __forceinline__ void walk_pair(uint2* A, const uint2* X, const uint4*
     for (int u = 0; u < 8; ++u)
       if (pos[u] >= 0) A[pos[u]] = val[u];
   }
+}
+
+// Ranks of the sorted new items X[0..m) (item r * 64 + lane in k[r], id[r])
+// in A[0..n): L[r] = #{A items before it} by binary search (the used bit of A
+// is ignored, so items being claimed meanwhile do not matter).  Then, unless
+// no new item lands below nout, the merge records (see sort_merge) for the
+// chunks from the smallest new item's position P0 up.  Returns P0.
+template <int R>
+__device__ __forceinline__ int rank_and_mark(const uint2* A, int n, int m, int nout,
+                                             const float* k, const uint32_t* id, int* L,
+                                             uint4* meta, int lane) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) L[r] = 0;
+  for (int step = 1 << (31 - __builtin_clz((unsigned)n)); step > 0; step >>= 1) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int i = L[r] + step - 1;
+      if (r * 64 + lane < m && i < n) {
+        const uint2 e = A[i];
+        if (kid_less(__uint_as_float(e.x), e.y & ~kUsedBit, k[r], id[r])) L[r] += step;
+      }
+    }
+  }
+  const int P0 = __builtin_amdgcn_readlane(L[0], 0);
+  if (P0 >= nout) return P0;
+  const int clo = P0 >> 6, chi = (nout - 1) >> 6;
+  for (int c = lane; c <= chi - clo; c += 64) meta[c] = make_uint4(0u, 0u, 0u, 0u);
+  WSYNC();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int j = r * 64 + lane;
+    const int p = L[r] + j;
+    if (j < m && p < nout)
+      atomicOr(reinterpret_cast<unsigned long long*>(&meta[(p >> 6) - clo]), 1ull << (p & 63));
+  }
+  WSYNC();
+  const uint64_t below_lane = (1ull << lane) - 1;
+  // new items below each chunk: an exclusive prefix over the chunks' counts,
+  // 64 chunks at a time, bit-sliced through ballots
+  int base = 0;
+  for (int g = 0; g <= chi - clo; g += 64) {
+    const int c = g + lane;
+    const bool in = c <= chi - clo;
+    const uint4 e = in ? meta[c] : make_uint4(0u, 0u, 0u, 0u);
+    const int cnt = __popc(e.x) + __popc(e.y);
+    int excl = base, tot = 0;
+#pragma unroll
+    for (int b = 0; b < 7; ++b) {
+      const uint64_t mb = __ballot((cnt >> b) & 1);
+      excl += __popcll(mb & below_lane) << b;
+      tot += __popcll(mb) << b;
+    }
+    if (in) meta[c].z = (uint32_t)excl;
+    base += tot;
+  }
+  WSYNC();
+  return P0;
 }
 
 // Sorts the m (<= 64 R) items of X[0..m) by (key, id), writes them back in
@@ -1204,22 +1262,46 @@ __device__ __forceinline__ int sort_merge(uint2* A, int n_, int C_, uint2* X, in
     if (j < m) X[j] = make_uint2(__float_as_uint(k[r]), id[r]);
   }
   WSYNC();
+  const int nout = min(C, n + m);
+  if constexpr (PAIR) {
+    // the second wave ranks the items and builds the merge records while
+    // this one claims the next picks; the ranks come back in link.lbuf
+    link.post(lane, kCmdRank, n, m, nout, (int)(X - link.items));
+    after_sort();
+    lap(4);
+    link.post(lane, kCmdNop);
+    const int* Lb = link.lbuf;
+    if (first) {
+      const int pa = __builtin_amdgcn_readfirstlane(nxt.x);
+      const int kx = __builtin_amdgcn_readfirstlane(nxt.y);
+      int fa = nout, fx = nout;
+      if (pa >= 0) {
+        int below = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int j = r * 64 + lane;
+          below += __popcll(__ballot(j < m && Lb[j < m ? j : 0] <= pa));
+        }
+        fa = pa + below;
+      }
+      if (kx < m) fx = __builtin_amdgcn_readfirstlane(Lb[kx]) + kx;
+      *first = min(nout, min(fa, fx));
+    }
+    const int P0 = __builtin_amdgcn_readfirstlane(Lb[0]);
+    before_walk();
+    lap(5);
+    if (P0 >= nout) return P0;
+    const int clo = P0 >> 6, chi = (nout - 1) >> 6;
+    link.post(lane, kCmdWalk, clo, chi, nout, (int)(X - link.items));
+    walk_pair(A, X, meta, clo, chi, nout, lane, 0);
+    link.post(lane, kCmdNop);  // the second wave's chunks are written
+    lap(6);
+    return P0;
+  }
   after_sort();
   lap(4);
   int L[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) L[r] = 0;
-  for (int step = 1 << (31 - __builtin_clz((unsigned)n)); step > 0; step >>= 1) {
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int i = L[r] + step - 1;
-      if (r * 64 + lane < m && i < n) {
-        const uint2 e = A[i];
-        if (kid_less(__uint_as_float(e.x), e.y & ~kUsedBit, k[r], id[r])) L[r] += step;
-      }
-    }
-  }
-  const int nout = min(C, n + m);
+  const int P0 = rank_and_mark<R>(A, n, m, nout, k, id, L, meta, lane);
   if (first) {
     const int pa = __builtin_amdgcn_readfirstlane(nxt.x);
     const int kx = __builtin_amdgcn_readfirstlane(nxt.y);
@@ -1237,52 +1319,14 @@ __device__ __forceinline__ int sort_merge(uint2* A, int n_, int C_, uint2* X, in
     }
     *first = min(nout, min(fa, fx));
   }
-  const int P0 = __builtin_amdgcn_readlane(L[0], 0);
   if (P0 >= nout) {
     before_walk();
     lap(5);
     return P0;
   }
   const int clo = P0 >> 6, chi = (nout - 1) >> 6;
-  for (int c = lane; c <= chi - clo; c += 64) meta[c] = make_uint4(0u, 0u, 0u, 0u);
-  WSYNC();
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int j = r * 64 + lane;
-    const int p = L[r] + j;
-    if (j < m && p < nout)
-      atomicOr(reinterpret_cast<unsigned long long*>(&meta[(p >> 6) - clo]), 1ull << (p & 63));
-  }
-  WSYNC();
-  const uint64_t below_lane = (1ull << lane) - 1;
-  // new items below each chunk: an exclusive prefix over the chunks' counts,
-  // 64 chunks at a time, bit-sliced through ballots
-  int base = 0;
-  for (int g = 0; g <= chi - clo; g += 64) {
-    const int c = g + lane;
-    const bool in = c <= chi - clo;
-    const uint4 e = in ? meta[c] : make_uint4(0u, 0u, 0u, 0u);
-    const int cnt = __popc(e.x) + __popc(e.y);
-    int excl = base, tot = 0;
-#pragma unroll
-    for (int b = 0; b < 7; ++b) {
-      const uint64_t mb = __ballot((cnt >> b) & 1);
-      excl += __popcll(mb & below_lane) << b;
-      tot += __popcll(mb) << b;
-    }
-    if (in) meta[c].z = (uint32_t)excl;
-    base += tot;
-  }
-  WSYNC();
   before_walk();
   lap(5);
-  if constexpr (PAIR) {
-    link.post(lane, kCmdWalk, clo, chi, nout, (int)(X - link.items));
-    walk_pair(A, X, meta, clo, chi, nout, lane, 0);
-    link.post(lane, kCmdNop);  // the second wave's chunks are written
-    lap(6);
-    return P0;
-  }
   for (int c = chi; c >= clo; c -= 8) {
     uint4 md[8];
 #pragma unroll
@@ -1644,11 +1688,38 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
   finish_query<D, FUNC, ST>(a, w, n, qi, k, lane, h, c);
 }
 
+// kCmdRank on the second wave: ranks of the sorted new items X[0..m) in
+// A[0..n) into w.cand (the first wave reads them back) and the merge records.
+// The first wave claims picks meanwhile, setting used bits in A and X, which
+// neither the ranks nor the ids read here depend on.
+template <int R>
+__device__ __forceinline__ void helper_rank(const WaveLds& w, const uint2* X, int n, int m, int nout,
+                                            int lane) {
+  float k[R];
+  uint32_t id[R];
+  int L[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int j = r * 64 + lane;
+    const uint2 v = j < m ? X[j] : make_uint2(0u, 0u);
+    k[r] = __uint_as_float(v.x);
+    id[r] = v.y & ~kUsedBit;
+  }
+  rank_and_mark<R>(w.A, n, m, nout, k, id, L, w.meta, lane);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int j = r * 64 + lane;
+    if (j < m) w.cand[j] = L[r];
+  }
+}
+
 // The second wave of a two-wave query (kModeBeamPair): carries out the
 // first wave's commands until it posts kCmdExit.
 //   kCmdDist (ncand, h0, topk, nsnap, ef, fallback id): distances of
 //     cand[h0..ncand) and the admitted ones as items at items[h0..); their
 //     number goes to ctl[16];
+//   kCmdRank (n, m, nout, offset of the new items in items): ranks and merge
+//     records of a merge pass (helper_rank);
 //   kCmdWalk (clo, chi, nout, offset of the new items in items): the lower
 //     half of each round of the rebuild walk.
 template <int D, int FUNC, int ST>
@@ -1673,6 +1744,14 @@ __device__ __forceinline__ void beam_helper(const SearchArgs& a, const WaveLds& 
       WSYNC();
       const int m1 = admit(w, h0, ncand, topk, nsnap, ef, lane, ltm, w.items + h0);
       if (lane == 0) w.ctl[16] = m1;
+    } else if (cmd == kCmdRank) {
+      const int n = __builtin_amdgcn_readfirstlane(c0.y);
+      const int m = __builtin_amdgcn_readfirstlane(c0.z);
+      const int nout = __builtin_amdgcn_readfirstlane(c0.w);
+      const uint2* X = w.items + __builtin_amdgcn_readfirstlane(c1.x);
+      if (m <= 64) helper_rank<1>(w, X, n, m, nout, lane);
+      else if (m <= 128) helper_rank<2>(w, X, n, m, nout, lane);
+      else helper_rank<4>(w, X, n, m, nout, lane);
     } else if (cmd == kCmdWalk) {
       walk_pair(w.A, w.items + __builtin_amdgcn_readfirstlane(c1.x), w.meta,
                 __builtin_amdgcn_readfirstlane(c0.y), __builtin_amdgcn_readfirstlane(c0.z),
@@ -1851,7 +1930,7 @@ __device__ __forceinline__ void search_body(const SearchArgs& a, int lds_wave_by
       return;
     }
   }
-  PairLink link{w.ctl, 0, w.items};
+  PairLink link{w.ctl, 0, w.items, w.cand};
   const int64_t slot = a.slot_base + (PAIR ? (int64_t)blockIdx.x
                                            : (int64_t)blockIdx.x * (blockDim.x >> 6) + wave);
   uint8_t* vis = a.ix.visited + slot * a.ix.vstride;

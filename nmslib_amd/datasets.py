@@ -73,6 +73,7 @@ WORKLOADS = {
     "halfnormal1m": (halfnormal_u8, 1_000_000, 128, 2, 3, 10_000),
     "decay2m": (decaying_normal, 2_000_000, 100, 4, 5, 8_192),
     "gmm10m": (gmm, 10_000_000, 96, 6, 7, 10_000),
+    "gmm2m": (gmm, 2_000_000, 96, 6, 7, 10_000),
 }
 
 

@@ -1,0 +1,117 @@
+"""The 10,000,000 x 96 negdotprod case at full size: an index built by the
+reference nmslib (M=24, efConstruction=200; its graph ships as a pack under
+workloads/, the vectors are regenerated from their seed: a 64-component
+Gaussian mixture, centres N(0,1), component sigma 0.3, normalised), 10,000
+queries from the same mixture, k=10.
+
+Exact mode equals the oracle row for row on a sample.  BF16 and per-vector-
+scaled FP8 e4m3 traversal, each followed by an exact FP32 rerank, are measured
+for recall@10 against the device brute force (itself checked against the FP64
+host oracle on a sample), and the distances they return, before and after the
+rerank, against FP64.  Skipped when the pack is absent (1 GB: too large to keep
+in the tree)."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.fixtures import recall_tie_aware
+from tests.gpu_util import load
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PACK = os.path.join(ROOT, "workloads", "gmm10m_M24_efc200.pack.npz")
+SAMPLE = slice(0, 10_000, 625)  # 16 queries
+
+
+@pytest.fixture(scope="module")
+def cfg4():
+    if not os.path.exists(PACK):
+        pytest.skip("10M negdotprod workload pack not present")
+    from nmslib_amd import datasets as D
+    from nmslib_amd import indexpack
+    pk = indexpack.load_pack(PACK)
+    # rows of the file are the generator's rows and labels are row numbers
+    assert np.array_equal(pk["ext_id"], np.arange(len(pk["ext_id"]), dtype=np.int32))
+    x = D.base("gmm10m")
+    path = os.path.join(tempfile.mkdtemp(prefix="nmsamd_cfg4_"), "gmm10m.hnsw")
+    indexpack.unpack(pk, x, path)
+    del pk
+    print("cfg4: index written", flush=True)
+    q = D.queries("gmm10m")
+    idx = load("gmm10m", path, "negdotprod")
+    print("cfg4: index loaded", flush=True)
+    gt, gd, _ = idx.bruteForce(q, 100)
+    # the device brute force against the FP64 host oracle on the sample
+    hi, hd = O.brute_force(x, q[SAMPLE], 10, "negdotprod")
+    assert recall_tie_aware(gt[SAMPLE], hi, hd, 10, rel=1e-6) == 1.0
+    print("cfg4: ground truth ready", flush=True)
+    yield path, idx, x, q, gt, gd
+    os.remove(path)
+
+
+def _fp64_dist(x, q, ids):
+    """-<x, q> in FP64 for each returned id (negdotprod)."""
+    out = np.zeros(ids.shape, np.float64)
+    for i in range(ids.shape[0]):
+        ok = ids[i] >= 0
+        out[i, ok] = -(x[ids[i, ok]].astype(np.float64) @ q[i].astype(np.float64))
+    return out
+
+
+def _run(idx, q, ef, beam, storage, rerank):
+    idx.setBeam(beam)
+    idx.setPrecision(storage, rerank)
+    idx.setQueryTimeParams({"efSearch": ef, "algoType": "v1merge"})
+    idx.knnQueryBatchArrays(q, 10)  # warm-up (first use builds the reduced-precision copy)
+    ids, dists, counts = idx.knnQueryBatchArrays(q, 10)
+    ms = idx.lastStats()["kernel_ms"]
+    idx.setBeam(0)
+    idx.setPrecision("fp32", 0)
+    return ids, dists, counts, ms
+
+
+def test_cfg4_exact_equals_oracle(cfg4):
+    path, idx, x, q, gt, gd = cfg4
+    ids, dists, counts, ms = _run(idx, q, 100, 0, "fp32", 0)
+    rec = recall_tie_aware(ids, gt, gd, 10)
+    print(f"cfg4 fp32 exact ef=100: {len(q) / ms * 1e3:.0f} q/s, recall@10 {rec:.4f}", flush=True)
+    oracle = O.OracleIndex(path)
+    oi, od, oc = oracle.search(q[SAMPLE], 10, 100, "v1merge")
+    np.testing.assert_array_equal(counts[SAMPLE], oc)
+    np.testing.assert_array_equal(ids[SAMPLE], oi)
+    np.testing.assert_array_equal(dists[SAMPLE].view(np.uint32), od.view(np.uint32))
+
+
+@pytest.mark.parametrize("storage", ["bf16", "fp8"])
+def test_cfg4_reduced_precision_with_rerank(cfg4, storage):
+    _, idx, x, q, gt, gd = cfg4
+    tol = 0.01 if storage == "bf16" else 0.05
+    for ef, beam in [(100, 0), (400, 0), (400, 8)]:
+        ex_ids, _, _, ex_ms = _run(idx, q, ef, beam, "fp32", 0)
+        ids, dists, counts, ms = _run(idx, q, ef, beam, storage, 2)
+        r_ex = recall_tie_aware(ex_ids, gt, gd, 10)
+        r_lo = recall_tie_aware(ids, gt, gd, 10)
+        # reranked distances are FP32 evaluations: their error against FP64
+        d64 = _fp64_dist(x, q[SAMPLE], ids[SAMPLE])
+        err_rr = float(np.max(np.abs(dists[SAMPLE].astype(np.float64) - d64)))
+        # the traversal's own distances (no rerank) against FP64
+        lids, ldists, _, _ = _run(idx, q[SAMPLE], ef, beam, storage, 0)
+        l64 = _fp64_dist(x, q[SAMPLE], lids)
+        err_tr = np.abs(ldists.astype(np.float64) - l64)
+        print(f"cfg4 {storage} beam={beam} ef={ef} rerank x2: {len(q) / ms * 1e3:.0f} q/s "
+              f"(fp32 {len(q) / ex_ms * 1e3:.0f} q/s), recall@10 {r_lo:.4f} (fp32 {r_ex:.4f}); "
+              f"|d - d64| after rerank max {err_rr:.2e}, traversal mean {err_tr.mean():.2e} "
+              f"max {err_tr.max():.2e}", flush=True)
+        assert r_lo >= r_ex - tol, (r_lo, r_ex)
+        # x and q have unit norm, so sum |x_i q_i| <= 1: an FP32 sum of 96
+        # products is within 96 * 2^-24 < 1e-5 of the exact value, and a
+        # coordinate rounded with relative error u moves the product sum by at
+        # most u (u = 2^-9 for BF16, 2^-4 for FP8 e4m3 in its normal range;
+        # the per-vector scale keeps the subnormal range's absolute error
+        # below 2^-10 * max|x| / 448 per coordinate)
+        assert err_rr <= 1e-5
+        assert err_tr.max() <= (2.0 ** -9 + 1e-5 if storage == "bf16" else 2.0 ** -4 + 1e-3)
