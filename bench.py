@@ -129,7 +129,7 @@ def _measured_traffic(path, form, ef):
     return int(m.group(1)) if m else None
 
 
-def serve_bench(idx, q, forms, rates, duration_s):
+def serve_bench(idx, q, forms, rates, duration_s, lanes=8, max_queries=1024):
     """The serving loop over the same resident index: Poisson arrivals, each
     request 1..256 queries with k in {1, 10, 100} and ef in 50..800, admitted
     into free lanes as they arrive (continuous batching)."""
@@ -140,11 +140,13 @@ def serve_bench(idx, q, forms, rates, duration_s):
         idx.setPrecision(storage, rerank)
         idx.setBeam(beam)
         idx.setQueryTimeParams({"algoType": "v1merge"})
-        srv = serving.Server(idx, lanes=4, max_queries=2048, max_ef=800, max_k=100, use_graph=True)
+        srv = serving.Server(idx, lanes=lanes, max_queries=max_queries, max_ef=800, max_k=100,
+                             use_graph=True)
         for rate in rates:
             reqs = serving.make_requests(len(q), rate, duration_s, seed=int(rate))
             st = serving.run_poisson(srv, q, reqs)
-            st.update({"form": name, "offered_qps": rate})
+            st.update({"form": name, "offered_qps": rate, "lanes": lanes,
+                       "max_queries_per_batch": max_queries})
             out.append({k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()})
             log(f"[bench] serve {name} offered={rate} -> {out[-1]}")
         srv.close()
@@ -285,6 +287,10 @@ def main():
     ap.add_argument("--serve", default="20000,100000",
                     help="offered query rates (q/s) of the serving run, '' to skip")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-lanes", type=int, default=8,
+                    help="concurrent batch lanes of the serving run")
+    ap.add_argument("--serve-batch", type=int, default=1024,
+                    help="most queries one lane launches at once")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="only a 1 GiB copy and one search launch (for rocprofv3 --pmc runs)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "hnsw_search_kernel_halfnormal1m_traffic.md"),
@@ -360,7 +366,7 @@ def main():
         rates = [float(r) for r in args.serve.split(",") if r]
         serve = serve_bench(idx, q, [("fp32 exact V1Merge", "fp32", 0, 0),
                                      ("bf16 beam16 + fp32 rerank x2", "bf16", 2, 16)],
-                            rates, args.serve_seconds)
+                            rates, args.serve_seconds, args.serve_lanes, args.serve_batch)
 
     cpu = None
     if not args.no_cpu:
