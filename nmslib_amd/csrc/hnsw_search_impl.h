@@ -207,6 +207,60 @@ __device__ __forceinline__ float lp_pair(float acc) {
   else return acc + p;
 }
 
+// BF16 storage: the query is rounded to BF16 too (round to nearest even,
+// as the stored vectors are), so a distance is packed products
+// (v_dot2_f32_bf16, two coordinates per instruction, FP32 accumulation).
+// The rounded query sits right after the FP32 one: word i of qb = coordinates
+// 2i and 2i+1, and qb[dstride / 2] = the sum of its squares.
+__device__ __forceinline__ const uint32_t* query_bf16(const float* qs, int dstride) {
+  return reinterpret_cast<const uint32_t*>(qs + dstride);
+}
+
+__device__ __forceinline__ uint32_t bf16_rne(float v) {
+  const uint32_t b = __float_as_uint(v);
+  return (b + 0x7fffu + ((b >> 16) & 1u)) >> 16;
+}
+
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ float dot2_bf16(uint32_t a, uint32_t b, float c) {
+  return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2_t, a),
+                                         __builtin_bit_cast(bf16x2_t, b), c, false);
+}
+
+// Writes the BF16 query (and its squared norm) after qs[0..dstride); one wave.
+__device__ __forceinline__ void make_query_bf16(float* qs, int dstride, int lane) {
+  uint32_t* qb = reinterpret_cast<uint32_t*>(qs + dstride);
+  float part = 0.f;
+  for (int i = lane; i < dstride / 2; i += 64) {
+    const uint32_t w = bf16_rne(qs[2 * i]) | (bf16_rne(qs[2 * i + 1]) << 16);
+    qb[i] = w;
+    part = dot2_bf16(w, w, part);
+  }
+  // fixed butterfly order: every wave of every kernel gets the same sum
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) part = part + __shfl_xor(part, o);
+  if (lane == 0) reinterpret_cast<float*>(qb)[dstride / 2] = part;
+}
+
+// The BF16 distance from the lane's partial products: x.q over its
+// coordinates (sxq) and, for l2, x.x (sxx); the lane pair's halves are added
+// and l2 is |x|^2 - 2 x.q + |q|^2 (clamped at 0).  Integer coordinates below
+// 256 keep every partial sum an exact integer below 2^24 (d <= 128), so there
+// the result equals the FP32 distance.
+template <int FUNC>
+__device__ __forceinline__ float dot_finish(float sxq, float sxx, const uint32_t* qb, int dstride) {
+  sxq = sxq + __shfl_xor(sxq, 1);
+  if constexpr (FUNC == kL2Sqr16 || FUNC == kL2Sqr) {
+    sxx = sxx + __shfl_xor(sxx, 1);
+    const float qq = reinterpret_cast<const float*>(qb)[dstride / 2];
+    const float s = (sxx - 2.0f * sxq) + qq;
+    return s < 0.0f ? 0.0f : s;
+  } else {
+    return finish<FUNC>(sxq);
+  }
+}
+
 // Reduced-precision distance (BF16 or per-vector-scaled FP8 e4m3 storage,
 // FP32 accumulation).  A lane pair splits every 16-element chunk into two
 // halves; the order is not the reference's (the stored values are not the
@@ -216,6 +270,35 @@ __device__ __forceinline__ float dist_pair_lp(const float* __restrict__ qs, cons
                                               int64_t node, int h, int drt) {
   const int d = D > 0 ? D : drt;
   const int q16 = d >> 4;
+  if constexpr (ST == kBF16 && FUNC <= kNegDot) {
+    constexpr bool L2 = FUNC == kL2Sqr16 || FUNC == kL2Sqr;
+    const uint32_t* qb = query_bf16(qs, ix.dstride);
+    const uint16_t* x = ix.vec16 + node * ix.dstride;
+    float sxq = 0.f, sxx = 0.f;
+#pragma unroll
+    for (int c = 0; c < q16; ++c) {
+      const int e0 = 16 * c + 8 * h;
+      const uint4 xr = *reinterpret_cast<const uint4*>(x + e0);
+      const uint4 qr = *reinterpret_cast<const uint4*>(qb + e0 / 2);
+      sxq = dot2_bf16(xr.x, qr.x, sxq);
+      sxq = dot2_bf16(xr.y, qr.y, sxq);
+      sxq = dot2_bf16(xr.z, qr.z, sxq);
+      sxq = dot2_bf16(xr.w, qr.w, sxq);
+      if constexpr (L2) {
+        sxx = dot2_bf16(xr.x, xr.x, sxx);
+        sxx = dot2_bf16(xr.y, xr.y, sxx);
+        sxx = dot2_bf16(xr.z, xr.z, sxx);
+        sxx = dot2_bf16(xr.w, xr.w, sxx);
+      }
+    }
+    for (int c = 16 * q16 + h; c < d; c += 2) {
+      const uint32_t xw = x[c];
+      const uint32_t qw = (qb[c >> 1] >> (16 * (c & 1))) & 0xffffu;
+      sxq = dot2_bf16(xw, qw, sxq);
+      if constexpr (L2) sxx = dot2_bf16(xw, xw, sxx);
+    }
+    return dot_finish<FUNC>(sxq, sxx, qb, ix.dstride);
+  }
   float acc = 0.f;
   float scale = 1.f;
   if constexpr (ST == kFP8) scale = ix.scale8[node];
@@ -332,9 +415,38 @@ __device__ __forceinline__ void vec_load(VecRegs<D, ST>& r, const DevIndex& ix, 
 
 template <int D, int FUNC, int ST>
 __device__ __forceinline__ float vec_dist(const VecRegs<D, ST>& r, const float* __restrict__ qs,
-                                          int h) {
+                                          int h, int dstride) {
   constexpr int Q16 = D / 16;
-  if constexpr (ST == kF32) {
+  if constexpr (ST == kBF16 && FUNC <= kNegDot) {
+    // the packed products of dist_pair_lp, in the same order
+    constexpr bool L2 = FUNC == kL2Sqr16 || FUNC == kL2Sqr;
+    const uint32_t* qb = query_bf16(qs, dstride);
+    float sxq = 0.f, sxx = 0.f;
+#pragma unroll
+    for (int c = 0; c < Q16; ++c) {
+      const uint4 xr = r.c[c];
+      const uint4 qr = *reinterpret_cast<const uint4*>(qb + 8 * c + 4 * h);
+      sxq = dot2_bf16(xr.x, qr.x, sxq);
+      sxq = dot2_bf16(xr.y, qr.y, sxq);
+      sxq = dot2_bf16(xr.z, qr.z, sxq);
+      sxq = dot2_bf16(xr.w, qr.w, sxq);
+      if constexpr (L2) {
+        sxx = dot2_bf16(xr.x, xr.x, sxx);
+        sxx = dot2_bf16(xr.y, xr.y, sxx);
+        sxx = dot2_bf16(xr.z, xr.z, sxx);
+        sxx = dot2_bf16(xr.w, xr.w, sxx);
+      }
+    }
+#pragma unroll
+    for (int t = 0; 2 * t + h < D % 16; ++t) {
+      const int c = 16 * Q16 + 2 * t + h;
+      const uint32_t xw = r.t[t];
+      const uint32_t qw = (qb[c >> 1] >> (16 * (c & 1))) & 0xffffu;
+      sxq = dot2_bf16(xw, qw, sxq);
+      if constexpr (L2) sxx = dot2_bf16(xw, xw, sxx);
+    }
+    return dot_finish<FUNC>(sxq, sxx, qb, dstride);
+  } else if constexpr (ST == kF32) {
     float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
 #pragma unroll
     for (int c = 0; c < Q16; ++c) {
@@ -452,7 +564,7 @@ __device__ __forceinline__ void dist_cands(const float* __restrict__ qs, const D
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         const int s = b + 32 * v + (lane >> 1);
-        const float dd = vec_dist<D, FUNC, ST>(r[v], qs, h);
+        const float dd = vec_dist<D, FUNC, ST>(r[v], qs, h, ix.dstride);
         if (s < ncand && h == 0) cdist[s] = dd;
       }
     }
@@ -484,7 +596,7 @@ struct DistRound {
 //   region Y: picks of the beam step -> filtered items -> sorted items
 struct WaveLds {
   uint2* A;        // [cap] sorted array: x = key bits, y = internal id | kUsedBit
-  float* qs;       // [dstride] query
+  float* qs;       // [dstride] query, then [dstride / 2 + 1] its BF16 form (query_bf16)
   uint32_t* htab;  // X [2 * ent] per-step dedupe table of the beam search
   int* cand;       // X [ent] fresh neighbours of this step (list order)
   float* cdist;    // X [ent] their distances
@@ -509,7 +621,7 @@ __device__ __forceinline__ WaveLds carve(unsigned char* base, int cap, int dstri
   WaveLds w;
   w.A = reinterpret_cast<uint2*>(base);
   w.qs = reinterpret_cast<float*>(w.A + ((cap + 1) & ~1));  // keeps 16-byte alignment
-  int* x = reinterpret_cast<int*>(w.qs + dstride);
+  int* x = reinterpret_cast<int*>(w.qs + 2 * dstride);  // query area: see query_bf16
   int* y = x + 2 * ent;
   w.htab = reinterpret_cast<uint32_t*>(x);
   w.cand = x;
@@ -1975,6 +2087,10 @@ __device__ __forceinline__ void search_body(const SearchArgs& a, int lds_wave_by
       inv = __shfl(inv, 0);
       if (inv != 0.f)
         for (int i = lane; i < dim; i += 64) w.qs[i] = w.qs[i] * inv;
+      WSYNC();
+    }
+    if constexpr (ST == kBF16) {
+      make_query_bf16(w.qs, a.ix.dstride, lane);
       WSYNC();
     }
     if (k <= 0 || a.ix.n == 0) {

@@ -364,11 +364,12 @@ int step_entries(const nmsamd_index* h) {
   return std::max(128, (w * (int)h->maxM0 + 63) / 64 * 64);
 }
 
-// sorted array + query + two scratch regions of 8 * ent bytes; beam mode
+// sorted array + query (FP32, then BF16 and its norm: 8 * dstride bytes) + two
+// scratch regions of 8 * ent bytes; beam mode
 // adds the merge's 16-byte record per 64 positions, the claimed next picks
 // (416 bytes) and the two-wave command words (96 bytes)
 int lds_wave_bytes(int cap, int dstride, int ent, bool beam) {
-  int b = ((cap + 1) & ~1) * 8 + dstride * 4 + 16 * ent;
+  int b = ((cap + 1) & ~1) * 8 + dstride * 8 + 16 * ent;
   if (beam) b += (cap / 64 + 2) * 16 + 416 + 96;
   return (b + 15) / 16 * 16;
 }
