@@ -542,7 +542,7 @@ __device__ __forceinline__ float vec_dist(const VecRegs<D, ST>& r, const float* 
 
 template <int ST>
 struct WideNV {  // vectors in flight per lane pair: about 128 registers of payload
-  static constexpr int value = ST == kF32 ? 2 : (ST == kBF16 ? 4 : 8);
+  static constexpr int value = ST == kF32 ? 2 : (ST == kBF16 ? 5 : 8);
 };
 
 // cdist[s] = distance of cand[s] for s < ncand (fallback: a valid node id used
