@@ -105,8 +105,8 @@ int nmsamd_set_beam(nmsamd_index* h, int beam);
 
 /* Waves per query in beam mode: 1, 2 (a block of two waves per query: the
  * second computes part of each step's distances and rebuilds half of the
- * sorted array), or 0 (default) = two where the LDS footprint leaves room for
- * fewer than two queries per SIMD.  Results do not depend on it.  No
+ * sorted array; at most three queries per CU), or 0 (default) = two where the
+ * LDS footprint limits a CU to three queries.  Results do not depend on it.  No
  * reference counterpart. */
 int nmsamd_set_beam_waves(nmsamd_index* h, int waves);
 

@@ -405,11 +405,15 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   // SIMD), else one.
   // With one wave per query, the wide beam kernel (registers for one wave
   // per SIMD) is taken where the footprint limits residency to that.
+  // Two waves per query run at most three queries per CU, the shape they are
+  // measured and tested in (variants at four queries per CU faulted the GPU
+  // on a 10,000-point index with the cause not found), and are chosen
+  // automatically only where LDS already limits a CU to three queries.
   const size_t fit = kLdsPerCu / lds;
-  const bool pair = beam && (h->beam_waves == 2 || (h->beam_waves == 0 && fit < 8));
+  const bool pair = beam && (h->beam_waves == 2 || (h->beam_waves == 0 && fit <= 3));
   const int wpb = pair ? 1 : (int)std::max<size_t>(1, std::min<size_t>(kWavesPerBlock, fit));
   const int per_cu = (int)std::max<size_t>(
-      1, std::min<size_t>(pair ? 4 : 8, kLdsPerCu / ((size_t)lds * wpb)));
+      1, std::min<size_t>(pair ? 3 : 8, kLdsPerCu / ((size_t)lds * wpb)));
   const int mode = !beam ? kModeAll
                          : (pair ? kModeBeamPair : (wpb * per_cu <= 4 ? kModeBeamWide : kModeBeam));
   int64_t max_blocks = (int64_t)h->cu_count * per_cu;
