@@ -76,3 +76,23 @@ def test_load_missing_file_is_runtime_error(tmp_path):
     idx = nmslib.init(space="l2")
     with pytest.raises(RuntimeError, match="Cannot open file"):
         idx.loadIndex(str(tmp_path / "missing.bin"))
+
+
+def test_extension_setters_validate_their_arguments():
+    # beam width, waves per beam query and precision: accepted ranges and the
+    # ValueError of anything outside them (no device needed)
+    idx = nmslib.init(space="l2")
+    for w in (0, 1, 16):
+        idx.setBeam(w)
+    for bad in (-1, 17):
+        with pytest.raises(ValueError, match="beam must be"):
+            idx.setBeam(bad)
+    for n in (0, 1, 2):
+        idx.setBeamWaves(n)
+    for bad in (-1, 3):
+        with pytest.raises(ValueError, match="beam waves must be"):
+            idx.setBeamWaves(bad)
+    idx.setPrecision("bf16", 2)
+    idx.setPrecision("fp32", 0)
+    with pytest.raises(ValueError):
+        idx.setPrecision("bf16", -1)
