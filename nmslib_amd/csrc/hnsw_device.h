@@ -95,7 +95,8 @@ struct SearchArgs {
   float* old_heap_key;       // SearchOld candidate heap, old_heap_cap per slot
   int32_t* old_heap_id;
   int64_t old_heap_cap;
-  int* err;                  // set to 1 when a SearchOld candidate heap overflows
+  int* err;                  // bit 1: a SearchOld candidate heap overflowed; bit 2: the
+                             // search state held a node id outside the index
 };
 
 // algoType (hnsw.cc:491-501); hybrid picks SearchOld for ef >= 1000 per query

@@ -767,6 +767,16 @@ __device__ __forceinline__ void greedy_descent(const DevIndex& ix, const float* 
 
 // position of the first item at or after p that is not expanded yet (n if
 // none): one chunk of 64, then four chunks per round trip
+// A node id read back from the LDS search state just before it addresses
+// global memory.  An id outside [0, n) could only come from a corrupted
+// state: it is reported through the error word (bit 2, raised as an error by
+// the host) and replaced by node 0, so no access leaves the index.
+__device__ __forceinline__ int checked_node(int id, int n, int* err) {
+  if ((unsigned)id < (unsigned)n) return id;
+  if (err) atomicOr(err, 2);
+  return 0;
+}
+
 __device__ __forceinline__ int first_unused(const uint2* A, int p, int n, int lane) {
   if (p < n) {
     const int i = p + lane;
@@ -811,7 +821,7 @@ __device__ __forceinline__ void finish_query(const SearchArgs& a, const WaveLds&
       for (int b = 0; b < R; b += 32) {
         const int s = b + (lane >> 1);
         const bool ok = s < R;
-        const int id = ok ? (int)(A[s].y & ~kUsedBit) : 0;
+        const int id = ok ? checked_node((int)(A[s].y & ~kUsedBit), (int)ix.n, a.err) : 0;
         const float dd = dist_f32<D, FUNC>(w.qs, ix.vec + (int64_t)id * ix.dstride, h, ix.dim);
         if (ok && h == 0) A[s] = make_uint2(__float_as_uint(dd), (uint32_t)id);
       }
@@ -851,7 +861,11 @@ __device__ __forceinline__ void finish_query(const SearchArgs& a, const WaveLds&
       int rank = start;
       for (int j = start; j < kk && akey(A, j) == key; ++j)
         rank += ((A[j].y & ~kUsedBit) < id) ? 1 : 0;
-      a.out_ids[row + rank] = ix.ext_id[id];
+      if (rank >= kk) {  // only an unsorted (corrupted) array gets here
+        if (a.err) atomicOr(a.err, 2);
+        rank = p;
+      }
+      a.out_ids[row + rank] = ix.ext_id[checked_node((int)id, (int)ix.n, a.err)];
       a.out_dists[row + rank] = key;
       if (a.out_internal) a.out_internal[row + rank] = (int32_t)id;
     } else if (p < k) {
@@ -1582,7 +1596,8 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
         const int e = lane + 64 * r;
         const int li = (e * mdiv) >> 16;
         const int sl = e - li * ix.maxM0;
-        if (li < pf_n) tn[r] = ix.l0[(int64_t)w.pfq[li].y * ix.l0stride + sl];
+        if (li < pf_n)
+          tn[r] = ix.l0[(int64_t)checked_node((int)w.pfq[li].y, (int)ix.n, a.err) * ix.l0stride + sl];
       }
     }
   };
@@ -1652,7 +1667,8 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
           const int e = lane + 64 * r;
           const int li = (e * mdiv) >> 16;
           const int sl = e - li * ix.maxM0;
-          if (li < nsel) t[r] = ix.l0[(int64_t)w.picks[li] * ix.l0stride + sl];
+          if (li < nsel)
+            t[r] = ix.l0[(int64_t)checked_node(w.picks[li], (int)ix.n, a.err) * ix.l0stride + sl];
         }
       }
 #pragma unroll

@@ -482,6 +482,8 @@ void finish_stats(nmsamd_index* h) {
   h->pending = false;
   int err = 0;
   hip_check(hipMemcpy(&err, h->d_work.p + 1, sizeof(int), hipMemcpyDeviceToHost), "status");
+  if (err & 2)
+    fail(NMSAMD_ERUNTIME, "internal error: the search state held a node id outside the index");
   if (err)
     fail(NMSAMD_ERUNTIME, "SearchOld candidate heap overflow (more than 64*ef admitted items)");
   float ms = 0;
