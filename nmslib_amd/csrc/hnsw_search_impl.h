@@ -545,14 +545,23 @@ struct WideNV {  // vectors in flight per lane pair: about 128 registers of payl
   static constexpr int value = ST == kF32 ? 2 : (ST == kBF16 ? 5 : 8);
 };
 
+// Vectors per lane pair and round of dist_cands: PATH 0 = one (dist_node),
+// 1 = WideNV (a wave with the register file of one wave per SIMD), 2 = two
+// BF16 / four FP8 vectors (two waves per SIMD; FP32 keeps one).
+template <int ST, int PATH>
+struct PathNV {
+  static constexpr int value =
+      PATH == 1 ? WideNV<ST>::value : (PATH == 2 ? (ST == kF32 ? 0 : (ST == kBF16 ? 2 : 4)) : 0);
+};
+
 // cdist[s] = distance of cand[s] for s < ncand (fallback: a valid node id used
 // by idle lane pairs so every shuffle sees all lanes)
 template <int D, int FUNC, int ST, int WIDE>
 __device__ __forceinline__ void dist_cands(const float* __restrict__ qs, const DevIndex& ix,
                                            const int* cand, float* cdist, int ncand, int lane,
                                            int h, int fallback) {
-  if constexpr (WIDE && D > 0 && D % 4 == 0 && (ST != kF32 || FUNC <= kNegDot)) {
-    constexpr int NV = WideNV<ST>::value;
+  constexpr int NV = PathNV<ST, WIDE>::value;
+  if constexpr (NV > 0 && D > 0 && D % 4 == 0 && (ST != kF32 || FUNC <= kNegDot)) {
     for (int b = 0; b < ncand; b += 32 * NV) {
       VecRegs<D, ST> r[NV];
 #pragma unroll
@@ -583,9 +592,9 @@ __device__ __forceinline__ void dist_cands(const float* __restrict__ qs, const D
 // pair, or NV per pair on the wide path).
 template <int D, int FUNC, int ST, int WIDE>
 struct DistRound {
+  static constexpr int NV = PathNV<ST, WIDE>::value;
   static constexpr int value =
-      (WIDE && D > 0 && D % 4 == 0 && (ST != kF32 || FUNC <= kNegDot)) ? 32 * WideNV<ST>::value
-                                                                        : 32;
+      (NV > 0 && D > 0 && D % 4 == 0 && (ST != kF32 || FUNC <= kNegDot)) ? 32 * NV : 32;
 };
 
 // Per-wave LDS: the sorted array, the query, two scratch regions of 8 * ent
@@ -1487,6 +1496,9 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
   const DevIndex& ix = a.ix;
   const int C = max(ef, k);
   const int E = a.ent;  // entries per step: W * maxM0 <= E <= 512
+  // distance path (PathNV): the wide kernel's, one vector per lane pair for
+  // the first wave of a two-wave query, else the two-waves-per-SIMD one
+  constexpr int DPATH = WIDE ? 1 : (PAIR ? 0 : 2);
   const int W = max(1, min(a.beam, E / ix.maxM0));
   const int erows = E >> 6;
   const uint32_t hslots = 2u * (uint32_t)E;
@@ -1736,13 +1748,13 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
     // wave's one-vector rounds are the slower ones there)
     int h0 = ncand;
     if constexpr (PAIR) {
-      constexpr int P0 = ST == kF32 ? 0 : DistRound<D, FUNC, ST, WIDE>::value;
-      constexpr int P1 = DistRound<D, FUNC, ST, true>::value;
+      constexpr int P0 = ST == kF32 ? 0 : DistRound<D, FUNC, ST, DPATH>::value;
+      constexpr int P1 = DistRound<D, FUNC, ST, 1>::value;
       const int r = (ncand + P0 + P1 - 1) / (P0 + P1);
       h0 = max(0, ncand - P1 * r);
       if (h0 < ncand) link.post(lane, kCmdDist, ncand, h0, __float_as_int(topk), nsnap, ef, cur);
     }
-    dist_cands<D, FUNC, ST, WIDE>(w.qs, ix, w.cand, w.cdist, h0, lane, h, cur);
+    dist_cands<D, FUNC, ST, DPATH>(w.qs, ix, w.cand, w.cdist, h0, lane, h, cur);
     WSYNC();
     lap(2);
     // keep what the snapshot admits; order does not matter (sorted next)
@@ -1868,7 +1880,7 @@ __device__ __forceinline__ void beam_helper(const SearchArgs& a, const WaveLds& 
       const int nsnap = __builtin_amdgcn_readfirstlane(c1.x);
       const int ef = __builtin_amdgcn_readfirstlane(c1.y);
       const int fb = __builtin_amdgcn_readfirstlane(c1.z);
-      dist_cands<D, FUNC, ST, true>(w.qs, a.ix, w.cand + h0, w.cdist + h0, ncand - h0, lane, h, fb);
+      dist_cands<D, FUNC, ST, 1>(w.qs, a.ix, w.cand + h0, w.cdist + h0, ncand - h0, lane, h, fb);
       WSYNC();
       const int m1 = admit(w, h0, ncand, topk, nsnap, ef, lane, ltm, w.items + h0);
       if (lane == 0) w.ctl[16] = m1;
