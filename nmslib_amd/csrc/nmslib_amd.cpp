@@ -867,6 +867,7 @@ struct ServerLane {
   int32_t *hk = nullptr, *hef = nullptr, *hids = nullptr, *hcounts = nullptr;
   float* hdists = nullptr;
   int64_t* hnq = nullptr;
+  int32_t* herr = nullptr;  // the batch's error word (SearchArgs::err)
   int64_t nq = 0;
   bool busy = false;
   hipGraph_t graph = nullptr;
@@ -892,7 +893,7 @@ struct nmsamd_server {
       if (L.t0) (void)hipEventDestroy(L.t0);
       if (L.stream) (void)hipStreamDestroy(L.stream);
       for (void* p : {(void*)L.hq, (void*)L.hk, (void*)L.hef, (void*)L.hids, (void*)L.hcounts,
-                      (void*)L.hdists, (void*)L.hnq})
+                      (void*)L.hdists, (void*)L.hnq, (void*)L.herr})
         if (p) (void)hipHostFree(p);
     }
   }
@@ -996,6 +997,7 @@ int nmsamd_server_open(nmsamd_index* h, int lanes, int max_queries, int max_ef, 
       hip_check(hipHostMalloc((void**)&L.hdists, sizeof(float) * max_queries * max_k), "hipHostMalloc");
       hip_check(hipHostMalloc((void**)&L.hcounts, sizeof(int32_t) * max_queries), "hipHostMalloc");
       hip_check(hipHostMalloc((void**)&L.hnq, sizeof(int64_t)), "hipHostMalloc");
+      hip_check(hipHostMalloc((void**)&L.herr, sizeof(int32_t)), "hipHostMalloc");
       L.slot_base = (int64_t)i * s->slots_per_lane;
       L.blocks = (int)std::max<int64_t>(1, s->slots_per_lane / s->wpb);
       if (s->use_graph) {
@@ -1035,6 +1037,7 @@ int nmsamd_server_submit(nmsamd_server* s, int lane, const float* q, int64_t nq,
     hip_check(hipMemcpyAsync(L.hids, L.ids.p, sizeof(int32_t) * out, hipMemcpyDeviceToHost, L.stream), "D2H");
     hip_check(hipMemcpyAsync(L.hdists, L.dists.p, sizeof(float) * out, hipMemcpyDeviceToHost, L.stream), "D2H");
     hip_check(hipMemcpyAsync(L.hcounts, L.counts.p, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, L.stream), "D2H");
+    hip_check(hipMemcpyAsync(L.herr, L.work.p + 1, sizeof(int32_t), hipMemcpyDeviceToHost, L.stream), "D2H");
     hip_check(hipEventRecord(L.done, L.stream), "hipEventRecord");
     L.nq = nq;
     L.busy = true;
@@ -1061,11 +1064,13 @@ int nmsamd_server_fetch(nmsamd_server* s, int lane, int32_t* ids, float* dists, 
     ServerLane& L = s->lanes[lane];
     if (!L.busy) fail(NMSAMD_EINVAL, "lane has no batch");
     hip_check(hipEventSynchronize(L.done), "server lane");
+    L.busy = false;
+    if (*L.herr & 2)
+      fail(NMSAMD_ERUNTIME, "internal error: the search state held a node id outside the index");
     const size_t out = (size_t)L.nq * s->max_k;
     std::memcpy(ids, L.hids, sizeof(int32_t) * out);
     std::memcpy(dists, L.hdists, sizeof(float) * out);
     std::memcpy(counts, L.hcounts, sizeof(int32_t) * L.nq);
-    L.busy = false;
   });
 }
 
