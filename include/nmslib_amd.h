@@ -52,6 +52,29 @@ int nmsamd_load_index(nmsamd_index* h, const char* filename, int load_data);
  * (hnsw.cc:776-806) does. */
 int nmsamd_save_index(nmsamd_index* h, const char* filename, int save_data);
 
+/* IndexWrapper::addDataPointBatch(data, ids) (nmslib.cc:342-344,
+ * readObjectVector :226-258): appends n dense float points of dimension dim
+ * ([n][dim] row-major) to the handle's data, as Objects with id ids[i] (or i,
+ * the row number within this batch, when ids is NULL) and label -1.
+ * *out_count (optional) receives n, the reference's return value. */
+int nmsamd_add_data_point_batch(nmsamd_index* h, const float* data, int64_t n, int64_t dim,
+                                const int32_t* ids, int64_t* out_count);
+
+/* IndexWrapper::createIndex(index_params, print_progress) (nmslib.cc:89-96 ->
+ * Hnsw::CreateIndex, hnsw.cc:183-470): builds the HNSW graph over the data
+ * added so far on the host CPU and keeps it as the optimized index, ready for
+ * saveIndex and for searching (it reaches the device on the first search).
+ * n strings "key=value": M (16), efConstruction (200), maxM (M), maxM0 (2M),
+ * mult (1/ln M), indexThreadQty (hardware threads), delaunay_type (2; 0, 1,
+ * 2 supported), post (0, 1, 2), searchMethod (ignored); unknown keys raise
+ * "Unknown parameters found!".  skip_optimized_index=1 and maxM0 above 64
+ * are rejected (the device path searches optimized indices with lists of at
+ * most 64).  With indexThreadQty=1 the graph equals the reference's: the
+ * level draws continue the calling thread's generator, seeded with 0 on each
+ * thread's first use, as the reference's do.  Resets query-time params
+ * (ef = 20).  print_progress is accepted and ignored. */
+int nmsamd_create_index(nmsamd_index* h, const char* const* kv, int n, int print_progress);
+
 /* setQueryTimeParams (nmslib.cc:599-607 -> Hnsw::SetQueryTimeParams,
  * hnsw.cc:472-507): n strings "key=value"; keys ef | efSearch, algoType
  * (hybrid | old | v1merge), searchMethod (ignored); unknown keys raise

@@ -55,6 +55,8 @@ SIGNATURES = {
     "nmsamd_free": (C.c_int, [_P]),
     "nmsamd_load_index": (C.c_int, [_P, C.c_char_p, C.c_int]),
     "nmsamd_save_index": (C.c_int, [_P, C.c_char_p, C.c_int]),
+    "nmsamd_add_data_point_batch": (C.c_int, [_P, _P, _I64, _I64, _P, C.POINTER(C.c_int64)]),
+    "nmsamd_create_index": (C.c_int, [_P, C.POINTER(C.c_char_p), C.c_int, C.c_int]),
     "nmsamd_set_query_time_params": (C.c_int, [_P, C.POINTER(C.c_char_p), C.c_int]),
     "nmsamd_knn_query_batch": (C.c_int, [_P, _P, _I64, _I64, _I32, _I32, _P, _P, _P]),
     "nmsamd_knn_query_batch_device": (C.c_int, [_P, _P, _I64, _I32, _P, _P, _P, _P]),
@@ -160,14 +162,45 @@ class FloatIndex:
         _check(load_library().nmsamd_save_index(self._h, os.fsencode(filename), int(bool(save_data))))
 
     def createIndex(self, index_params=None, print_progress=False):
-        raise RuntimeError("createIndex is not part of the device query path: build the index "
-                           "with nmslib and load it with loadIndex")
+        """Builds the HNSW graph over the added data (nmslib.cc:89-96); the
+        parameters are those of Hnsw::CreateIndex (hnsw.cc:187-205)."""
+        kv = _params_to_kv(index_params)
+        arr = (C.c_char_p * max(1, len(kv)))(*[s.encode() for s in kv])
+        _check(load_library().nmsamd_create_index(self._h, arr, len(kv), int(bool(print_progress))))
+        self._dim = self.info()["dim"]
 
     def addDataPoint(self, id, data):
-        raise RuntimeError("addDataPoint is not part of the device query path")
+        """Appends one point with object id `id`; returns its position
+        (nmslib.cc:337-340)."""
+        v = np.ascontiguousarray(np.asarray(data, dtype=np.float32).reshape(1, -1))
+        ids = np.array([id], np.int32)
+        _check(load_library().nmsamd_add_data_point_batch(self._h, _ptr(v), 1, v.shape[1], _ptr(ids),
+                                                          None))
+        return len(self) - 1
 
     def addDataPointBatch(self, data, ids=None):
-        raise RuntimeError("addDataPointBatch is not part of the device query path")
+        """Appends the rows of `data` (readObjectVector, nmslib.cc:226-258):
+        a 2-D array force-cast to C-order float32, or a list of vectors; point
+        i gets id ids[i], or i when ids is None.  Returns the number added."""
+        if isinstance(data, list):
+            a = np.array([np.asarray(v, dtype=np.float32) for v in data], dtype=np.float32)
+        else:
+            a = np.asarray(data)
+            if a.ndim != 2:
+                raise RuntimeError("data must be a 2d array")
+            a = np.ascontiguousarray(a, dtype=np.float32)
+        if a.ndim != 2:
+            raise RuntimeError("data must be a 2d array")
+        idp = None
+        if ids is not None:
+            idv = np.ascontiguousarray(np.asarray(ids), dtype=np.int32)
+            if idv.shape != (a.shape[0],):
+                raise ValueError("ids must hold one id per data point")
+            idp = _ptr(idv)
+        out = C.c_int64(0)
+        _check(load_library().nmsamd_add_data_point_batch(self._h, _ptr(a), a.shape[0], a.shape[1],
+                                                          idp, C.byref(out)))
+        return int(out.value)
 
     # -- query-time parameters -------------------------------------------
     def setQueryTimeParams(self, params=None):

@@ -13,8 +13,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libnmslib_amd.so")
 SOURCES = ["search_f32.hip", "search_bf16.hip", "search_fp8.hip", "launch.hip",
-           "lowp_convert.hip", "bruteforce.hip", "nmslib_amd.cpp"]
+           "lowp_convert.hip", "bruteforce.hip", "nmslib_amd.cpp", "hnsw_build.cpp"]
+# host-only sources: the index builder (x86 SSE, no device code)
+HOST_SOURCES = {"hnsw_build.cpp"}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
+HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-pthread", "-Wall"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wall", "-Wno-unused-function"]
@@ -28,7 +32,9 @@ def build(verbose: bool = False) -> str:
     for src in SOURCES:
         obj = os.path.join(builddir, src + ".o")
         cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
-        if src.endswith(".cpp"):
+        if src in HOST_SOURCES:
+            cmd = [CXX, *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        elif src.endswith(".cpp"):
             cmd = [HIPCC, *FLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
@@ -46,7 +52,7 @@ def build(verbose: bool = False) -> str:
     if failed:
         raise RuntimeError(f"hipcc failed on {failed}")
     tmp = OUT + ".tmp"
-    subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs],
+    subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs],
                    check=True)
     os.replace(tmp, OUT)
     return OUT

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <deque>
@@ -20,9 +21,11 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/nmslib_amd.h"
+#include "hnsw_build.h"
 #include "hnsw_device.h"
 
 namespace nmsamd {
@@ -107,6 +110,7 @@ struct nmsamd_index {
   bool collect = false;
   // file header (hnsw.cc:779-788)
   bool loaded = false;
+  bool on_device = false;  // a built index is uploaded on its first search
   uint32_t n = 0;
   uint64_t elem = 0, off_l0 = 0, off_data = 0, maxM = 0, maxM0 = 0, method = 0;
   int32_t maxlevel = 0, func = 0;
@@ -174,7 +178,9 @@ void ensure_device(nmsamd_index* h) {
   h->cu_count = prop.multiProcessorCount;
 }
 
-void parse_and_upload(nmsamd_index* h, const std::string& path) {
+// Reads an optimized index file (LoadOptimizedIndex, hnsw.cc:1025-1074) into
+// the handle's host copy.
+void read_index_file(nmsamd_index* h, const std::string& path) {
   std::ifstream in(path, std::ios::binary);
   if (!in) fail(NMSAMD_EIO, "Cannot open file '" + path + "' for reading");
   in.exceptions(std::ios::badbit | std::ios::failbit);
@@ -223,9 +229,16 @@ void parse_and_upload(nmsamd_index* h, const std::string& path) {
   } catch (const std::ios_base::failure& e) {
     fail(NMSAMD_EIO, std::string("error reading index '") + path + "': " + e.what());
   }
+}
+
+// Validates the host copy (ids in range, list counts within capacity, the
+// entry point on the top level, the space's distance function) and notes
+// whether any level-0 list repeats an id.
+void check_index(nmsamd_index* h) {
+  if (h->maxM0 > (uint64_t)kMaxM0 || h->maxM > (uint64_t)kMaxM0)
+    fail(NMSAMD_ERUNTIME, "maxM0 " + std::to_string(h->maxM0) + " exceeds the device limit of " +
+                              std::to_string(kMaxM0));
   if (h->n > 0 && h->enter >= h->n) fail(NMSAMD_ERUNTIME, "entry point out of range");
-  const auto elem_node = h->func;  // kept for message clarity
-  (void)elem_node;
   if (h->expected_func == 1 && h->func != kL2Sqr16 && h->func != kL2Sqr)
     fail(NMSAMD_ERUNTIME, "index distance function does not match space l2");
   if (h->expected_func == kNormCos && h->func != kNormCos)
@@ -236,20 +249,11 @@ void parse_and_upload(nmsamd_index* h, const std::string& path) {
     fail(NMSAMD_ERUNTIME, "index distance function does not match space l1");
   if (h->expected_func == kLInf && h->func != kLInf)
     fail(NMSAMD_ERUNTIME, "index distance function does not match space linf");
-
-  // device layout (hnsw_device.h)
   const int64_t n = h->n;
-  h->dstride = (h->dim + 3) / 4 * 4;
-  h->l0stride = ((int)h->maxM0 + 31) / 32 * 32;
-  std::vector<float> vec((size_t)std::max<int64_t>(n, 1) * h->dstride, 0.f);
-  std::vector<int32_t> l0((size_t)std::max<int64_t>(n, 1) * h->l0stride, -1);
-  std::vector<int32_t> ext((size_t)std::max<int64_t>(n, 1), 0);
   bool dups = false;
   std::vector<int32_t> tmp;
   for (int64_t i = 0; i < n; ++i) {
     const char* e = h->block.data() + i * h->elem;
-    std::memcpy(&ext[i], e, 4);
-    std::memcpy(&vec[i * h->dstride], e + 16, 4 * (size_t)h->dim);
     int32_t cnt;
     std::memcpy(&cnt, e + h->off_l0, 4);
     if (cnt < 0 || (uint64_t)cnt > h->maxM0) fail(NMSAMD_ERUNTIME, "corrupt level-0 list count");
@@ -257,7 +261,6 @@ void parse_and_upload(nmsamd_index* h, const std::string& path) {
     tmp.assign(ids, ids + cnt);
     for (int32_t v : tmp)
       if (v < 0 || v >= n) fail(NMSAMD_ERUNTIME, "level-0 neighbour id out of range");
-    std::memcpy(&l0[i * h->l0stride], ids, 4 * (size_t)cnt);
     std::sort(tmp.begin(), tmp.end());
     if (std::adjacent_find(tmp.begin(), tmp.end()) != tmp.end()) dups = true;
   }
@@ -279,7 +282,24 @@ void parse_and_upload(nmsamd_index* h, const std::string& path) {
     if (lev_enter < h->maxlevel) fail(NMSAMD_ERUNTIME, "entry point level below maxlevel");
   }
   h->has_dups = dups;
+}
 
+// Lays the checked host copy out for the device (hnsw_device.h) and uploads it.
+void upload_index(nmsamd_index* h) {
+  const int64_t n = h->n;
+  h->dstride = (h->dim + 3) / 4 * 4;
+  h->l0stride = ((int)h->maxM0 + 31) / 32 * 32;
+  std::vector<float> vec((size_t)std::max<int64_t>(n, 1) * h->dstride, 0.f);
+  std::vector<int32_t> l0((size_t)std::max<int64_t>(n, 1) * h->l0stride, -1);
+  std::vector<int32_t> ext((size_t)std::max<int64_t>(n, 1), 0);
+  for (int64_t i = 0; i < n; ++i) {
+    const char* e = h->block.data() + i * h->elem;
+    std::memcpy(&ext[i], e, 4);
+    std::memcpy(&vec[i * h->dstride], e + 16, 4 * (size_t)h->dim);
+    int32_t cnt;
+    std::memcpy(&cnt, e + h->off_l0, 4);
+    std::memcpy(&l0[i * h->l0stride], e + h->off_l0 + 4, 4 * (size_t)cnt);
+  }
   ensure_device(h);
   hipStream_t s = h->stream;
   h->d_vec.upload(vec.data(), vec.size(), s);
@@ -295,10 +315,18 @@ void parse_and_upload(nmsamd_index* h, const std::string& path) {
   h->d_vec16.release();
   h->d_vec8.release();
   h->d_scale8.release();
+  h->d_xnorm.release();
   h->d_visited.release();
   h->d_epoch.release();
   h->slots = 0;
-  h->loaded = true;
+  h->on_device = true;
+}
+
+// Entry points that search: an index must be loaded or built; a built one
+// reaches the device on its first use.
+void require_index(nmsamd_index* h) {
+  if (!h->loaded) fail(NMSAMD_EINVAL, "Must call createIndex or loadIndex before this method");
+  if (!h->on_device) upload_index(h);
 }
 
 // Visited-mark arrays for `slots` resident waves.
@@ -494,6 +522,47 @@ void finish_stats(nmsamd_index* h) {
               "stats");
 }
 
+
+// AnyParams from "key=value" strings (params.h:45-120): one '=' per string,
+// no repeated key.
+std::vector<std::pair<std::string, std::string>> parse_params(const char* const* kv, int n) {
+  std::vector<std::pair<std::string, std::string>> params;
+  for (int i = 0; i < n; ++i) {
+    const std::string s = kv[i] ? kv[i] : "";
+    const size_t eq = s.find('=');
+    if (eq == std::string::npos || s.find('=', eq + 1) != std::string::npos)
+      fail(NMSAMD_ERUNTIME, "Wrong format of an argument: '" + s +
+                                "' should be in the format: <Name>=<Value>");
+    const std::string name = s.substr(0, eq);
+    for (auto& p : params)
+      if (p.first == name) fail(NMSAMD_ERUNTIME, "Duplicate parameter: " + name);
+    params.emplace_back(name, s.substr(eq + 1));
+  }
+  return params;
+}
+
+// AnyParamManager::ConvertStrToValue (params.h:295-305): the whole string
+// must convert; the message names the type as typeid does (m, i, d).
+size_t to_size(const std::string& v) {
+  std::stringstream ss(v);
+  size_t out = 0;
+  if (!(ss >> out) || !ss.eof() || v.find('-') != std::string::npos)
+    fail(NMSAMD_ERUNTIME, "Failed to convert value '" + v + "' from type: m");
+  return out;
+}
+int to_int(const std::string& v) {
+  std::stringstream ss(v);
+  int out = 0;
+  if (!(ss >> out) || !ss.eof()) fail(NMSAMD_ERUNTIME, "Failed to convert value '" + v + "' from type: i");
+  return out;
+}
+double to_double(const std::string& v) {
+  std::stringstream ss(v);
+  double out = 0;
+  if (!(ss >> out) || !ss.eof()) fail(NMSAMD_ERUNTIME, "Failed to convert value '" + v + "' from type: d");
+  return out;
+}
+
 template <class F>
 int guarded(F&& f) {
   try {
@@ -567,7 +636,11 @@ int nmsamd_load_index(nmsamd_index* h, const char* filename, int load_data) {
         fail(NMSAMD_EIO, "error reading '" + dat + "': " + e.what());
       }
     }
-    parse_and_upload(h, filename);
+    h->on_device = false;
+    read_index_file(h, filename);
+    check_index(h);
+    upload_index(h);
+    h->loaded = true;
     h->ef = 20;  // ResetQueryTimeParams (nmslib.cc:110-112)
     h->algo = kHybrid;
   });
@@ -610,21 +683,102 @@ int nmsamd_save_index(nmsamd_index* h, const char* filename, int save_data) {
   });
 }
 
+int nmsamd_add_data_point_batch(nmsamd_index* h, const float* data, int64_t n, int64_t dim,
+                                const int32_t* ids, int64_t* out_count) {
+  return guarded([&] {
+    if (!h || (n > 0 && !data)) fail(NMSAMD_EINVAL, "null argument");
+    if (n < 0) fail(NMSAMD_EINVAL, "the number of points must be non-negative");
+    if (dim <= 0) fail(NMSAMD_EINVAL, "data points need a positive dimension");
+    std::lock_guard<std::mutex> g(h->mu);
+    // Object buffer (object.h:40-104): i32 id, i32 label (-1), u64 datalength, data
+    const uint64_t len = 4 * (uint64_t)dim;
+    for (int64_t i = 0; i < n; ++i) {
+      std::vector<char> obj(16 + len);
+      const int32_t id = ids ? ids[i] : (int32_t)i;
+      const int32_t label = -1;
+      std::memcpy(obj.data(), &id, 4);
+      std::memcpy(obj.data() + 4, &label, 4);
+      std::memcpy(obj.data() + 8, &len, 8);
+      std::memcpy(obj.data() + 16, data + i * dim, len);
+      h->data_objs.push_back(std::move(obj));
+    }
+    if (out_count) *out_count = n;
+  });
+}
+
+int nmsamd_create_index(nmsamd_index* h, const char* const* kv, int nkv, int print_progress) {
+  (void)print_progress;
+  return guarded([&] {
+    if (!h) fail(NMSAMD_EINVAL, "null handle");
+    std::lock_guard<std::mutex> g(h->mu);
+    // Hnsw::CreateIndex parameters and defaults (hnsw.cc:187-205)
+    BuildParams p;
+    bool has_maxM = false, has_maxM0 = false, has_mult = false;
+    int skip_optimized = 0;
+    p.threads = std::max(1u, std::thread::hardware_concurrency());
+    for (auto& kvp : parse_params(kv, nkv)) {
+      const std::string& k = kvp.first;
+      const std::string& v = kvp.second;
+      if (k == "M") p.M = to_size(v);
+      else if (k == "maxM") { p.maxM = to_size(v); has_maxM = true; }
+      else if (k == "maxM0") { p.maxM0 = to_size(v); has_maxM0 = true; }
+      else if (k == "efConstruction") p.efConstruction = to_size(v);
+      else if (k == "indexThreadQty") {
+        p.threads = to_size(v);
+        if (p.threads == 0) p.threads = std::max(1u, std::thread::hardware_concurrency());
+      }
+      else if (k == "mult") { p.mult = to_double(v); has_mult = true; }
+      else if (k == "delaunay_type") p.delaunay = to_int(v);
+      else if (k == "post") p.post = to_int(v);
+      else if (k == "skip_optimized_index") skip_optimized = to_int(v);
+      else if (k == "searchMethod") (void)to_int(v);
+      else fail(NMSAMD_ERUNTIME, "Unknown parameters found!");
+    }
+    if (!has_maxM) p.maxM = p.M;
+    if (!has_maxM0) p.maxM0 = p.M * 2;
+    if (!has_mult) p.mult = 1 / std::log(1.0 * p.M);
+    if (skip_optimized)
+      fail(NMSAMD_ERUNTIME, "skip_optimized_index=1 keeps the regular (non-optimized) index, "
+                            "which the device path does not search");
+    if (p.maxM0 > (size_t)kMaxM0 || p.maxM > (size_t)kMaxM0)
+      fail(NMSAMD_EINVAL, "maxM0 " + std::to_string(p.maxM0) + " exceeds the device limit of " +
+                              std::to_string(kMaxM0));
+    if (h->data_objs.empty()) fail(NMSAMD_EINVAL, "no data points: call addDataPointBatch first");
+    const int space = h->expected_func == 1 ? (int)kBuildL2 : h->expected_func;
+    BuiltIndex b;
+    try {
+      build_hnsw(space, h->data_objs, p, b);
+    } catch (const std::invalid_argument& e) {
+      fail(NMSAMD_EINVAL, e.what());
+    }
+    h->loaded = false;
+    h->on_device = false;
+    h->n = b.n;
+    h->elem = b.elem;
+    h->off_l0 = b.off_l0;
+    h->off_data = 0;
+    h->maxlevel = b.maxlevel;
+    h->enter = b.enter;
+    h->maxM = b.maxM;
+    h->maxM0 = b.maxM0;
+    h->func = b.func;
+    h->method = 3;  // searchMethod of every optimized index (hnsw.cc:379)
+    h->dim = (int)((b.off_l0 - 16) / 4);
+    h->block = std::move(b.block);
+    h->sizemass = std::move(b.sizemass);
+    h->upper = std::move(b.upper);
+    h->upper_off = std::move(b.upper_off);
+    check_index(h);
+    h->loaded = true;
+    h->ef = 20;  // CreateIndex resets the query-time parameters (hnsw.cc:207)
+    h->algo = kHybrid;
+  });
+}
+
 int nmsamd_set_query_time_params(nmsamd_index* h, const char* const* kv, int n) {
   return guarded([&] {
     if (!h) fail(NMSAMD_EINVAL, "null handle");
-    std::vector<std::pair<std::string, std::string>> params;
-    for (int i = 0; i < n; ++i) {
-      const std::string s = kv[i] ? kv[i] : "";
-      const size_t eq = s.find('=');
-      if (eq == std::string::npos || s.find('=', eq + 1) != std::string::npos)
-        fail(NMSAMD_ERUNTIME, "Wrong format of an argument: '" + s +
-                                  "' should be in the format: <Name>=<Value>");
-      const std::string name = s.substr(0, eq);
-      for (auto& p : params)
-        if (p.first == name) fail(NMSAMD_ERUNTIME, "Duplicate parameter: " + name);
-      params.emplace_back(name, s.substr(eq + 1));
-    }
+    const auto params = parse_params(kv, n);
     auto has = [&](const char* name) {
       for (auto& p : params) if (p.first == name) return true;
       return false;
@@ -632,23 +786,13 @@ int nmsamd_set_query_time_params(nmsamd_index* h, const char* const* kv, int n) 
     if (has("ef") && has("efSearch"))
       fail(NMSAMD_ERUNTIME,
            "The user shouldn't specify parameters ef and efSearch at the same time (they are synonyms)");
-    auto to_size = [](const std::string& v) {
-      std::stringstream ss(v);
-      size_t out = 0;
-      if (!(ss >> out) || !ss.eof() || v.find('-') != std::string::npos)
-        fail(NMSAMD_ERUNTIME, "Failed to convert value '" + v + "' from type: m");
-      return out;
-    };
     size_t ef = 20;
     Algo algo = kHybrid;
     for (auto& p : params) {
       if (p.first == "ef" || p.first == "efSearch") {
         ef = to_size(p.second);
       } else if (p.first == "searchMethod") {
-        std::stringstream ss(p.second);
-        int tmp;
-        if (!(ss >> tmp) || !ss.eof())
-          fail(NMSAMD_ERUNTIME, "Failed to convert value '" + p.second + "' from type: i");
+        (void)to_int(p.second);
       } else if (p.first == "algoType") {
         const std::string t = to_lower(p.second);
         if (t == "v1merge") algo = kV1Merge;
@@ -670,7 +814,7 @@ int nmsamd_knn_query_batch(nmsamd_index* h, const float* q, int64_t nq, int64_t 
   return guarded([&] {
     if (!h) fail(NMSAMD_EINVAL, "null handle");
     std::lock_guard<std::mutex> g(h->mu);
-    if (!h->loaded) fail(NMSAMD_EINVAL, "Must call createIndex or loadIndex before this method");
+    require_index(h);
     if (nq < 0 || k < 0) fail(NMSAMD_EINVAL, "nq and k must be non-negative");
     if (dim != h->dim)
       fail(NMSAMD_EINVAL, "query dimension " + std::to_string(dim) + " != index dimension " +
@@ -700,7 +844,7 @@ int nmsamd_knn_query_batch_device(nmsamd_index* h, const float* d_q, int64_t nq,
   return guarded([&] {
     if (!h) fail(NMSAMD_EINVAL, "null handle");
     std::lock_guard<std::mutex> g(h->mu);
-    if (!h->loaded) fail(NMSAMD_EINVAL, "Must call createIndex or loadIndex before this method");
+    require_index(h);
     if (nq < 0 || k <= 0) fail(NMSAMD_EINVAL, "nq must be non-negative and k positive");
     if (nq == 0) return;
     hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : h->stream;
@@ -714,7 +858,7 @@ int nmsamd_knn_query_batch_uneven(nmsamd_index* h, const float* q, int64_t nq, i
   return guarded([&] {
     if (!h) fail(NMSAMD_EINVAL, "null handle");
     std::lock_guard<std::mutex> g(h->mu);
-    if (!h->loaded) fail(NMSAMD_EINVAL, "Must call createIndex or loadIndex before this method");
+    require_index(h);
     if (dim != h->dim) fail(NMSAMD_EINVAL, "query dimension does not match the index");
     if (nq <= 0) return;
     int mef = 0, mk = 0;
@@ -945,7 +1089,7 @@ int nmsamd_server_open(nmsamd_index* h, int lanes, int max_queries, int max_ef, 
                        int use_graph, nmsamd_server** out) {
   return guarded([&] {
     if (!h || !out) fail(NMSAMD_EINVAL, "null argument");
-    if (!h->loaded) fail(NMSAMD_EINVAL, "Must call createIndex or loadIndex before this method");
+    require_index(h);
     if (lanes < 1 || lanes > 16 || max_queries < 1 || max_ef < 1 || max_k < 1)
       fail(NMSAMD_EINVAL, "lanes in 1..16 and positive max_queries / max_ef / max_k required");
     if (max_ef >= 1000 && h->algo != kV1Merge)
@@ -1110,7 +1254,7 @@ extern "C" int nmsamd_brute_force(nmsamd_index* h, const float* q, int64_t nq, i
   return guarded([&] {
     if (!h) fail(NMSAMD_EINVAL, "null handle");
     std::lock_guard<std::mutex> g(h->mu);
-    if (!h->loaded) fail(NMSAMD_EINVAL, "Must call createIndex or loadIndex before this method");
+    require_index(h);
     if (dim != h->dim) fail(NMSAMD_EINVAL, "query dimension does not match the index");
     if (h->func == kL1 || h->func == kLInf)
       fail(NMSAMD_EINVAL, "brute force on matrix cores covers l2, cosinesimil and negdotprod");

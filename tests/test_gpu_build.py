@@ -1,0 +1,80 @@
+"""Searching an index built by createIndex on the host: the graph reaches the
+device on the first search, and the device's answers equal the oracle's on
+the saved file (ids and FP32 distances bit-exact), for a fresh build, after a
+save / load round trip with its data, and after a handle that held a loaded
+index builds a new one."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import nmslib_amd as nm
+from nmslib_amd import datasets as D
+from oracle import oracle as O
+from tests import fixtures as F
+from tests.gpu_util import require_gpu
+
+pytestmark = pytest.mark.gpu
+
+_TMP = tempfile.mkdtemp(prefix="nmsamd_gbuild_")
+
+
+def _same_as_oracle(idx, path, q, k, ef, algo="hybrid"):
+    idx.setQueryTimeParams({"efSearch": ef, "algoType": algo})
+    ids, dists, counts = idx.knnQueryBatchArrays(q, k)
+    oi, od, oc = O.OracleIndex(path).search(q, k, ef, algo)
+    np.testing.assert_array_equal(counts, oc)
+    np.testing.assert_array_equal(ids, oi)
+    np.testing.assert_array_equal(dists.view(np.uint32), od.view(np.uint32))
+    return ids
+
+
+@pytest.mark.parametrize("space, gen, dim, M", [("l2", D.uniform, 64, 16),
+                                                ("cosinesimil", D.decaying_normal, 100, 32),
+                                                ("negdotprod", D.gmm, 96, 24),
+                                                ("l1", D.uniform, 40, 12)])
+def test_built_index_searches_on_device(space, gen, dim, M):
+    require_gpu()
+    x = gen(20_000, dim, 41)
+    q = gen(300, dim, 42)
+    idx = nm.init(method="hnsw", space=space)
+    idx.addDataPointBatch(x)
+    idx.createIndex({"M": M, "efConstruction": 100, "indexThreadQty": 8})
+    path = os.path.join(_TMP, f"{space}.hnsw")
+    idx.saveIndex(path, save_data=True)
+    print(f"built {space} {x.shape}", flush=True)
+    for k, ef, algo in [(10, 64, "hybrid"), (1, 16, "v1merge"), (50, 200, "old")]:
+        _same_as_oracle(idx, path, q, k, ef, algo)
+    if space != "l1":  # the matrix-core brute force covers l2, cosine, negdotprod
+        gt, gd = O.brute_force(x, q, 10, space)
+        idx.setQueryTimeParams({"efSearch": 100})
+        ids, _, _ = idx.knnQueryBatchArrays(q, 10)
+        assert F.recall_tie_aware(ids, gt, gd, 10, rel=1e-5) >= 0.9
+    # the saved file, with its data, loads into a new handle with the same answers
+    again = nm.init(method="hnsw", space=space)
+    again.loadIndex(path, load_data=True)
+    assert len(again) == len(x)
+    _same_as_oracle(again, path, q, 10, 64)
+
+
+def test_rebuild_replaces_a_loaded_index():
+    require_gpu()
+    name = "u128_l2"
+    idx = nm.init(method="hnsw", space="l2")
+    idx.loadIndex(F.index_path(name))
+    q = F.results(name)["queries"]
+    idx.setQueryTimeParams({"efSearch": 32})
+    idx.knnQueryBatchArrays(q, 10)
+    idx.bruteForce(q[:20], 5)  # leaves the first index's norms on the device
+    x = D.uniform(5_000, 128, 43)
+    idx.addDataPointBatch(x)
+    idx.createIndex({"M": 12, "efConstruction": 80})
+    path = os.path.join(_TMP, "rebuilt.hnsw")
+    idx.saveIndex(path)
+    assert idx.info()["n"] == len(x)
+    _same_as_oracle(idx, path, q, 10, 48)
+    # the brute force now answers over the new vectors
+    ids, dists, _ = idx.bruteForce(q[:20], 5)
+    oi, od = O.brute_force(x, q[:20], 5, "l2")
+    assert F.recall_tie_aware(ids, oi, od, 5, rel=1e-5) == 1.0
