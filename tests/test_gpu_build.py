@@ -46,11 +46,14 @@ def test_built_index_searches_on_device(space, gen, dim, M):
     print(f"built {space} {x.shape}", flush=True)
     for k, ef, algo in [(10, 64, "hybrid"), (1, 16, "v1merge"), (50, 200, "old")]:
         _same_as_oracle(idx, path, q, k, ef, algo)
-    if space != "l1":  # the matrix-core brute force covers l2, cosine, negdotprod
+    if space != "l1":
+        # recall@10 at ef 400 against the exact answer; graphs the reference
+        # builds from the same data with 8 threads measure 0.991 (l2) and
+        # 0.9993 (cosinesimil) through the same search
         gt, gd = O.brute_force(x, q, 10, space)
-        idx.setQueryTimeParams({"efSearch": 100})
+        idx.setQueryTimeParams({"efSearch": 400})
         ids, _, _ = idx.knnQueryBatchArrays(q, 10)
-        assert F.recall_tie_aware(ids, gt, gd, 10, rel=1e-5) >= 0.9
+        assert F.recall_tie_aware(ids, gt, gd, 10, rel=1e-5) >= 0.97
     # the saved file, with its data, loads into a new handle with the same answers
     again = nm.init(method="hnsw", space=space)
     again.loadIndex(path, load_data=True)
