@@ -555,10 +555,10 @@ void build_hnsw(int space, const std::vector<std::vector<char>>& objs, const Bui
   out.maxM0 = p.maxM0;
   out.maxlevel = g.maxlevel();
   out.enter = (uint32_t)g.enter();
-  switch (space) {
-    case kBuildL2: out.func = dim % 16 == 0 ? 1 : 2; break;
-    case kBuildCosine: out.func = 3; break;
-    case kBuildNegDot: out.func = 4; break;
+  switch (space) {  // optimized: hnsw.cc:381-405; regular: the space's own distance (7, 8, 9)
+    case kBuildL2: out.func = p.optimized ? (dim % 16 == 0 ? 1 : 2) : 7; break;
+    case kBuildCosine: out.func = p.optimized ? 3 : 8; break;
+    case kBuildNegDot: out.func = p.optimized ? 4 : 9; break;
     case kBuildL1: out.func = 5; break;
     default: out.func = 6; break;
   }
@@ -568,7 +568,7 @@ void build_hnsw(int space, const std::vector<std::vector<char>>& objs, const Bui
   for (size_t i = 0; i < n; ++i) {
     char* e = out.block.data() + i * out.elem;
     std::memcpy(e, objs[i].data(), data_bytes);
-    if (space == kBuildCosine) normalize_vect(reinterpret_cast<float*>(e + 16), dim);
+    if (space == kBuildCosine && p.optimized) normalize_vect(reinterpret_cast<float*>(e + 16), dim);
     const Node& nd = g.node(i);
     const auto& l0 = nd.links[0];
     if (l0.size() > p.maxM0) throw std::runtime_error("internal error: level-0 list over capacity");

@@ -29,6 +29,8 @@ struct BuildParams {
   double mult = 0;        // level multiplier (default 1/ln M)
   int delaunay = 2;       // neighbour selection: 0 closest, 1 heuristic 1, 2 heuristic 2
   int post = 0;           // 1, 2: merge with a graph built in reverse order
+  bool optimized = true;  // false (skip_optimized_index=1): keep the regular form, whose
+                          // vectors are not normalised and whose distances are the space's
 };
 
 // An optimized index image: the file's header fields, its level-0 block

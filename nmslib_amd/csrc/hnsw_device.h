@@ -27,8 +27,16 @@
 
 namespace nmsamd {
 
-// distance function codes of the optimized index (include/method/hnsw.h:50-58)
-enum DistFunc : int { kL2Sqr16 = 1, kL2Sqr = 2, kNormCos = 3, kNegDot = 4, kL1 = 5, kLInf = 6 };
+// distance function codes of the optimized index (include/method/hnsw.h:50-58),
+// then the spaces' own distances that a regular (non-optimized) index is
+// searched with (hnsw.cc:1076-1300 through Query::DistanceObjLeft: the data
+// point is the left operand): L2NormSIMD (the square root of the squared
+// distance), CosineSimilarity of the raw vectors and -ScalarProductSIMD, all in
+// the four-lane SSE order.  l1 and linf use codes 5 and 6 in both formats.
+enum DistFunc : int {
+  kL2Sqr16 = 1, kL2Sqr = 2, kNormCos = 3, kNegDot = 4, kL1 = 5, kLInf = 6,
+  kSpaceL2 = 7, kSpaceCos = 8, kSpaceNegDot = 9
+};
 // vector storage used while traversing
 enum Storage : int { kF32 = 0, kBF16 = 1, kFP8 = 2 };
 

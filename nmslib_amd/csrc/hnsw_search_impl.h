@@ -173,11 +173,80 @@ __device__ __forceinline__ float dist_pair_l1inf(const float* __restrict__ qs,
   }
 }
 
+// The spaces' distances of a regular index (L2SqrSIMD + sqrt, distcomp_lp.cc:
+// 304-369; NormScalarProductSIMD, distcomp_scalar.cc:83-168, inside
+// CosineSimilarity :267-271; ScalarProductSIMD :194-245) by a lane pair, the
+// data point x being the left operand: lane h keeps SSE lanes 2h and 2h+1 of
+// every group of 4 floats; the four lanes are summed left to right, then the
+// tail is added element by element in float.  Both lanes return the same value.
+template <int FUNC>
+__device__ __forceinline__ float dist_pair_space(const float* __restrict__ qs,
+                                                 const float* __restrict__ x, int h, int d) {
+  const int q4 = d >> 2;
+  float a0 = 0.f, a1 = 0.f, n0 = 0.f, n1 = 0.f, m0 = 0.f, m1 = 0.f;
+  for (int g = 0; g < q4; ++g) {
+    const float2 xv = *reinterpret_cast<const float2*>(x + 4 * g + 2 * h);
+    const float2 qv = *reinterpret_cast<const float2*>(qs + 4 * g + 2 * h);
+    if constexpr (FUNC == kSpaceL2) {
+      const float t0 = xv.x - qv.x, t1 = xv.y - qv.y;
+      a0 = a0 + t0 * t0;
+      a1 = a1 + t1 * t1;
+    } else {
+      a0 = a0 + xv.x * qv.x;
+      a1 = a1 + xv.y * qv.y;
+      if constexpr (FUNC == kSpaceCos) {
+        n0 = n0 + xv.x * xv.x;
+        n1 = n1 + xv.y * xv.y;
+        m0 = m0 + qv.x * qv.x;
+        m1 = m1 + qv.y * qv.y;
+      }
+    }
+  }
+  auto lanes4 = [h](float v0, float v1) {  // SSE lanes 0 + 1 + 2 + 3, left to right
+    const float p0 = __shfl_xor(v0, 1), p1 = __shfl_xor(v1, 1);
+    const float l0 = h ? p0 : v0, l1 = h ? p1 : v1;
+    const float u0 = h ? v0 : p0, u1 = h ? v1 : p1;
+    float s = l0 + l1;
+    s = s + u0;
+    return s + u1;
+  };
+  float s = lanes4(a0, a1);
+  if constexpr (FUNC == kSpaceL2) {
+    for (int c = 4 * q4; c < d; ++c) {
+      const float t = x[c] - qs[c];
+      s = s + t * t;
+    }
+    return sqrtf(s);
+  } else if constexpr (FUNC == kSpaceNegDot) {
+    for (int c = 4 * q4; c < d; ++c) s = s + x[c] * qs[c];
+    return -s;
+  } else {
+    float nx = lanes4(n0, n1), nq = lanes4(m0, m1);
+    for (int c = 4 * q4; c < d; ++c) {
+      s = s + x[c] * qs[c];
+      nx = nx + x[c] * x[c];
+      nq = nq + qs[c] * qs[c];
+    }
+    // 0 when either norm is below 2 FLT_MIN, else the clamped normalised
+    // product; the distance is max(0, 1 - that)
+    const float eps = 2.0f * 1.17549435e-38f;
+    float c = 0.0f;
+    if (!(nx < eps || nq < eps)) {
+      const float r = s / sqrtf(nx) / sqrtf(nq);
+      const float lo = r < 1.0f ? r : 1.0f;
+      c = -1.0f < lo ? lo : -1.0f;
+    }
+    const float v = 1.0f - c;
+    return 0.0f < v ? v : 0.0f;
+  }
+}
+
 // FP32 distance in the reference's order for the function
 template <int D, int FUNC>
 __device__ __forceinline__ float dist_f32(const float* __restrict__ qs, const float* __restrict__ x,
                                           int h, int drt) {
   if constexpr (FUNC == kL1 || FUNC == kLInf) return dist_pair_l1inf<FUNC>(qs, x, h, D > 0 ? D : drt);
+  else if constexpr (FUNC >= kSpaceL2) return dist_pair_space<FUNC>(qs, x, h, D > 0 ? D : drt);
   else return dist_pair_f32<D, FUNC>(qs, x, h, drt);
 }
 

@@ -110,7 +110,12 @@ struct nmsamd_index {
   bool collect = false;
   // file header (hnsw.cc:779-788)
   bool loaded = false;
-  bool on_device = false;  // a built index is uploaded on its first search
+  bool on_device = false;  // uploaded on the first search when no device was visible before
+  // regular (non-optimized) index (hnsw.cc:810-841, 943-991): searched with the
+  // space's own distance; saved back in its own format
+  bool regular = false;
+  uint64_t reg_M = 0;          // M_ of its header
+  int32_t file_maxlevel = 0;   // maxlevel_ of its header (the search starts at the entry's level)
   uint32_t n = 0;
   uint64_t elem = 0, off_l0 = 0, off_data = 0, maxM = 0, maxM0 = 0, method = 0;
   int32_t maxlevel = 0, func = 0;
@@ -178,8 +183,148 @@ void ensure_device(nmsamd_index* h) {
   h->cu_count = prop.multiProcessorCount;
 }
 
-// Reads an optimized index file (LoadOptimizedIndex, hnsw.cc:1025-1074) into
-// the handle's host copy.
+// Space code -> distance code of a regular index (the space's own function).
+int regular_func(int expected_func) {
+  switch (expected_func) {
+    case 1: return kSpaceL2;
+    case kNormCos: return kSpaceCos;
+    case kNegDot: return kSpaceNegDot;
+    default: return expected_func;  // l1, linf: the same functions as the optimized codes
+  }
+}
+
+// Lays a graph given as per-node lists out like an optimized index image over
+// the handle's data points (vectors as given: the regular search does not
+// normalise).  lists[i][L] = node i's list at level L.
+void image_from_lists(nmsamd_index* h, const std::vector<std::vector<std::vector<int32_t>>>& lists) {
+  const size_t n = lists.size();
+  if (n != h->data_objs.size())
+    fail(NMSAMD_ERUNTIME, "The number of stored elements " + std::to_string(n) +
+                              " doesn't match the number of data points " +
+                              std::to_string(h->data_objs.size()) + "! Did you forget to re-load data?");
+  uint64_t len = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const auto& o = h->data_objs[i];
+    uint64_t dl = 0;
+    if (o.size() >= 16) std::memcpy(&dl, o.data() + 8, 8);
+    if (o.size() < 16 || dl + 16 != o.size() || dl == 0 || dl % 4 != 0 || (i && dl != len))
+      fail(NMSAMD_ERUNTIME, "data point " + std::to_string(i) + " is not a dense float vector "
+                            "of the index's dimension");
+    len = dl;
+  }
+  h->dim = (int)(len / 4);
+  h->off_l0 = 16 + len;
+  h->off_data = 0;
+  h->elem = h->off_l0 + 4 * (h->maxM0 + 1);
+  h->block.assign(n * h->elem, 1);
+  h->sizemass.assign(n, 0);
+  h->upper.clear();
+  h->upper_off.assign(n, -1);
+  for (size_t i = 0; i < n; ++i) {
+    char* e = h->block.data() + i * h->elem;
+    std::memcpy(e, h->data_objs[i].data(), h->off_l0);
+    const auto& l0 = lists[i][0];
+    if (l0.size() > h->maxM0) fail(NMSAMD_ERUNTIME, "level-0 list longer than maxM0");
+    const int32_t cnt = (int32_t)l0.size();
+    std::memcpy(e + h->off_l0, &cnt, 4);
+    if (cnt) std::memcpy(e + h->off_l0 + 4, l0.data(), 4 * l0.size());
+    const size_t levels = lists[i].size() - 1;
+    if (levels) {
+      h->sizemass[i] = (uint32_t)(levels * (h->maxM + 1) * 4);
+      h->upper_off[i] = (int32_t)h->upper.size();
+      for (size_t L = 1; L <= levels; ++L) {
+        const auto& lst = lists[i][L];
+        if (lst.size() > h->maxM) fail(NMSAMD_ERUNTIME, "upper-level list longer than maxM");
+        h->upper.push_back((int32_t)lst.size());
+        h->upper.insert(h->upper.end(), lst.begin(), lst.end());
+        h->upper.resize(h->upper.size() + (h->maxM - lst.size()), 0);
+      }
+    }
+  }
+  h->n = (uint32_t)n;
+  h->func = regular_func(h->expected_func);
+  h->method = 0;
+  h->regular = true;
+  // baseSearchAlgorithm*: the descent starts at the entry point's own level
+  h->maxlevel = n ? (int32_t)(lists[h->enter].size() - 1) : 0;
+}
+
+// LoadRegularIndexBin (hnsw.cc:943-991), after the format flag: the header,
+// then per node its level and, for levels 0..level, [count, ids].  The index
+// holds no vectors: the handle's data points must be loaded or added first.
+void read_regular_body(nmsamd_index* h, std::istream& in) {
+  uint32_t n = 0, enter = 0;
+  int32_t maxlevel = 0;
+  uint64_t M = 0, maxM = 0, maxM0 = 0;
+  rd(in, n);
+  rd(in, maxlevel);
+  rd(in, enter);
+  rd(in, M);
+  rd(in, maxM);
+  rd(in, maxM0);
+  if (n != h->data_objs.size())
+    fail(NMSAMD_ERUNTIME, "The number of stored elements " + std::to_string(n) +
+                              " doesn't match the number of data points " +
+                              std::to_string(h->data_objs.size()) + "! Did you forget to re-load data?");
+  if (maxM0 > (uint64_t)kMaxM0 || maxM > (uint64_t)kMaxM0)
+    fail(NMSAMD_ERUNTIME, "maxM0 " + std::to_string(maxM0) + " exceeds the device limit of " +
+                              std::to_string(kMaxM0));
+  if (n && enter >= n) fail(NMSAMD_ERUNTIME, "entry point out of range");
+  std::vector<std::vector<std::vector<int32_t>>> lists(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    uint32_t level = 0;
+    rd(in, level);
+    if (level > 64) fail(NMSAMD_ERUNTIME, "corrupt node level");
+    lists[i].resize(level + 1);
+    for (uint32_t L = 0; L <= level; ++L) {
+      uint32_t qty = 0;
+      rd(in, qty);
+      if (qty > (L ? maxM : maxM0)) fail(NMSAMD_ERUNTIME, "corrupt friend list size");
+      auto& f = lists[i][L];
+      f.resize(qty);
+      if (qty) in.read(reinterpret_cast<char*>(f.data()), 4 * (size_t)qty);
+      for (int32_t v : f)
+        if (v < 0 || (uint32_t)v >= n)
+          fail(NMSAMD_ERUNTIME, "Invalid friendId = " + std::to_string(v) +
+                                    " for node id: " + std::to_string(i));
+    }
+  }
+  h->maxM = maxM;
+  h->maxM0 = maxM0;
+  h->enter = enter;
+  h->reg_M = M;
+  h->file_maxlevel = maxlevel;
+  image_from_lists(h, lists);
+}
+
+// SaveRegularIndexBin (hnsw.cc:810-841) from the handle's image.
+void write_regular(const nmsamd_index* h, std::ostream& out) {
+  wr(out, (uint32_t)0);
+  wr(out, h->n);
+  wr(out, h->file_maxlevel);
+  wr(out, h->enter);
+  wr(out, h->reg_M);
+  wr(out, h->maxM);
+  wr(out, h->maxM0);
+  for (uint32_t i = 0; i < h->n; ++i) {
+    const char* e = h->block.data() + (size_t)i * h->elem;
+    const uint32_t levels = h->sizemass[i] / (uint32_t)(4 * (h->maxM + 1));
+    wr(out, levels);
+    int32_t cnt;
+    std::memcpy(&cnt, e + h->off_l0, 4);
+    wr(out, (uint32_t)cnt);
+    out.write(e + h->off_l0 + 4, 4 * (size_t)cnt);
+    for (uint32_t L = 1; L <= levels; ++L) {
+      const int32_t* lst = h->upper.data() + h->upper_off[i] + (h->maxM + 1) * (L - 1);
+      wr(out, (uint32_t)lst[0]);
+      out.write(reinterpret_cast<const char*>(lst + 1), 4 * (size_t)lst[0]);
+    }
+  }
+}
+
+
+// Reads an index file (LoadIndex, hnsw.cc:993-1074: the optimized format, or
+// the regular one over the handle's data points) into the handle's host copy.
 void read_index_file(nmsamd_index* h, const std::string& path) {
   std::ifstream in(path, std::ios::binary);
   if (!in) fail(NMSAMD_EIO, "Cannot open file '" + path + "' for reading");
@@ -187,9 +332,12 @@ void read_index_file(nmsamd_index* h, const std::string& path) {
   try {
     uint32_t flag = 0;
     rd(in, flag);
-    if (!flag)
-      fail(NMSAMD_ERUNTIME,
-           "the regular (non-optimized) HNSW index format is not supported by the device path");
+    h->regular = false;
+    if (!flag) {
+      read_regular_body(h, in);
+      return;
+    }
+    h->method = 3;
     rd(in, h->n);
     rd(in, h->elem);
     rd(in, h->off_l0);
@@ -239,11 +387,11 @@ void check_index(nmsamd_index* h) {
     fail(NMSAMD_ERUNTIME, "maxM0 " + std::to_string(h->maxM0) + " exceeds the device limit of " +
                               std::to_string(kMaxM0));
   if (h->n > 0 && h->enter >= h->n) fail(NMSAMD_ERUNTIME, "entry point out of range");
-  if (h->expected_func == 1 && h->func != kL2Sqr16 && h->func != kL2Sqr)
+  if (h->expected_func == 1 && h->func != kL2Sqr16 && h->func != kL2Sqr && h->func != kSpaceL2)
     fail(NMSAMD_ERUNTIME, "index distance function does not match space l2");
-  if (h->expected_func == kNormCos && h->func != kNormCos)
+  if (h->expected_func == kNormCos && h->func != kNormCos && h->func != kSpaceCos)
     fail(NMSAMD_ERUNTIME, "index distance function does not match space cosinesimil");
-  if (h->expected_func == kNegDot && h->func != kNegDot)
+  if (h->expected_func == kNegDot && h->func != kNegDot && h->func != kSpaceNegDot)
     fail(NMSAMD_ERUNTIME, "index distance function does not match space negdotprod");
   if (h->expected_func == kL1 && h->func != kL1)
     fail(NMSAMD_ERUNTIME, "index distance function does not match space l1");
@@ -414,6 +562,9 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   const bool any_old = !beam && (h->algo == kOld || (h->algo == kHybrid && max_ef >= 1000));
   int cap = std::max(1, std::max(max_ef, max_k));
   if (any_old) cap = std::max(cap, max_ef + 1 + max_k);
+  if (h->regular && h->storage != kF32)
+    fail(NMSAMD_EINVAL, "reduced-precision traversal needs an optimized index (a regular one is "
+                        "searched with the space's own FP32 distance)");
   const bool rerank = h->storage != kF32 && h->rerank_mult > 0;
   if (rerank) {  // staging room for the re-sorted candidates
     const int r = std::min(std::max(max_ef, max_k), std::max(max_k, h->rerank_mult * max_k));
@@ -616,8 +767,8 @@ int nmsamd_load_index(nmsamd_index* h, const char* filename, int load_data) {
     if (!h || !filename) fail(NMSAMD_EINVAL, "null argument");
     std::lock_guard<std::mutex> g(h->mu);
     h->loaded = false;
-    h->data_objs.clear();
-    if (load_data) {
+    if (load_data) {  // without load_data the points added so far stay (nmslib.cc:103-107)
+      h->data_objs.clear();
       const std::string dat = std::string(filename) + ".dat";
       std::ifstream in(dat, std::ios::binary);
       if (!in) fail(NMSAMD_EIO, "Cannot open file '" + dat + "' for reading");
@@ -639,7 +790,8 @@ int nmsamd_load_index(nmsamd_index* h, const char* filename, int load_data) {
     h->on_device = false;
     read_index_file(h, filename);
     check_index(h);
-    upload_index(h);
+    int devices = 0;  // upload now when a device is visible, else on the first search
+    if (hipGetDeviceCount(&devices) == hipSuccess && devices > 0) upload_index(h);
     h->loaded = true;
     h->ef = 20;  // ResetQueryTimeParams (nmslib.cc:110-112)
     h->algo = kHybrid;
@@ -662,6 +814,11 @@ int nmsamd_save_index(nmsamd_index* h, const char* filename, int save_data) {
     }
     std::ofstream out(filename, std::ios::binary);
     if (!out) fail(NMSAMD_EIO, std::string("Cannot open file '") + filename + "' for writing");
+    if (h->regular) {
+      write_regular(h, out);
+      if (!out) fail(NMSAMD_EIO, std::string("error writing '") + filename + "'");
+      return;
+    }
     wr(out, (uint32_t)1);
     wr(out, h->n);
     wr(out, h->elem);
@@ -737,9 +894,7 @@ int nmsamd_create_index(nmsamd_index* h, const char* const* kv, int nkv, int pri
     if (!has_maxM) p.maxM = p.M;
     if (!has_maxM0) p.maxM0 = p.M * 2;
     if (!has_mult) p.mult = 1 / std::log(1.0 * p.M);
-    if (skip_optimized)
-      fail(NMSAMD_ERUNTIME, "skip_optimized_index=1 keeps the regular (non-optimized) index, "
-                            "which the device path does not search");
+    p.optimized = skip_optimized == 0;
     if (p.maxM0 > (size_t)kMaxM0 || p.maxM > (size_t)kMaxM0)
       fail(NMSAMD_EINVAL, "maxM0 " + std::to_string(p.maxM0) + " exceeds the device limit of " +
                               std::to_string(kMaxM0));
@@ -753,6 +908,9 @@ int nmsamd_create_index(nmsamd_index* h, const char* const* kv, int nkv, int pri
     }
     h->loaded = false;
     h->on_device = false;
+    h->regular = !p.optimized;
+    h->reg_M = p.M;
+    h->file_maxlevel = b.maxlevel;
     h->n = b.n;
     h->elem = b.elem;
     h->off_l0 = b.off_l0;
@@ -762,7 +920,7 @@ int nmsamd_create_index(nmsamd_index* h, const char* const* kv, int nkv, int pri
     h->maxM = b.maxM;
     h->maxM0 = b.maxM0;
     h->func = b.func;
-    h->method = 3;  // searchMethod of every optimized index (hnsw.cc:379)
+    h->method = p.optimized ? 3 : 0;  // searchMethod: 3 for every optimized index (hnsw.cc:373)
     h->dim = (int)((b.off_l0 - 16) / 4);
     h->block = std::move(b.block);
     h->sizemass = std::move(b.sizemass);
@@ -1256,8 +1414,9 @@ extern "C" int nmsamd_brute_force(nmsamd_index* h, const float* q, int64_t nq, i
     std::lock_guard<std::mutex> g(h->mu);
     require_index(h);
     if (dim != h->dim) fail(NMSAMD_EINVAL, "query dimension does not match the index");
-    if (h->func == kL1 || h->func == kLInf)
-      fail(NMSAMD_EINVAL, "brute force on matrix cores covers l2, cosinesimil and negdotprod");
+    if (h->func == kL1 || h->func == kLInf || h->regular)
+      fail(NMSAMD_EINVAL, "brute force on matrix cores covers optimized l2, cosinesimil and "
+                          "negdotprod indices");
     if (h->dstride > 128) fail(NMSAMD_EINVAL, "brute force supports dimensions up to 128");
     if (k < 1 || k > 128) fail(NMSAMD_EINVAL, "brute force k must be in 1..128");
     if (nq <= 0) return;

@@ -12,6 +12,9 @@ KernelFn pick_kernel_f32(int dim, int func, int mode) {
     case kNegDot: return kernel_of<D, kNegDot, kF32, W>();        \
     case kL1: return kernel_of<0, kL1, kF32, W>();                 \
     case kLInf: return kernel_of<0, kLInf, kF32, W>();             \
+    case kSpaceL2: return kernel_of<0, kSpaceL2, kF32, W>();       \
+    case kSpaceCos: return kernel_of<0, kSpaceCos, kF32, W>();     \
+    case kSpaceNegDot: return kernel_of<0, kSpaceNegDot, kF32, W>(); \
     default: return nullptr;                                           \
   }
 #define NMS_DIMS(W)                \

@@ -170,7 +170,6 @@ def test_multithreaded_build_invariants_and_recall():
     ({"M": 8, "foo": 1}, RuntimeError, "Unknown parameters found!"),
     (["M=abc"], RuntimeError, "Failed to convert value 'abc' from type: m"),
     (["M"], RuntimeError, "Wrong format of an argument"),
-    ({"skip_optimized_index": 1}, RuntimeError, "regular"),
     ({"delaunay_type": 3}, ValueError, "delaunay_type 3"),
     ({"post": 3}, ValueError, "post"),
     ({"M": 40}, ValueError, "device limit"),
