@@ -42,14 +42,19 @@ int nmsamd_init(const char* space, const char* method, nmsamd_index** out);
 int nmsamd_free(nmsamd_index* h);
 
 /* IndexWrapper::loadIndex(filename, load_data) (nmslib.cc:98-113,
- * Hnsw::LoadIndex hnsw.cc:993-1074): reads the optimized index file, uploads
- * the graph and vectors to HBM, resets query-time params (ef = 20).
- * load_data also reads filename + ".dat" (space.cc:61-86). */
+ * Hnsw::LoadIndex hnsw.cc:993-1074): reads the index file, uploads the graph
+ * and vectors to HBM (on the first search when no device is visible yet),
+ * resets query-time params (ef = 20).  load_data first replaces the data
+ * points with filename + ".dat" (space.cc:61-86); without it the points added
+ * so far stay.  Both formats load: the optimized one, and the regular one
+ * (hnsw.cc:943-991), which holds no vectors and is laid over the data points
+ * (their number must match) and searched with the space's own distance. */
 int nmsamd_load_index(nmsamd_index* h, const char* filename, int load_data);
 
 /* IndexWrapper::saveIndex(filename, save_data) (nmslib.cc:115-125): writes
- * the optimized format byte-for-byte as Hnsw::SaveOptimizedIndex
- * (hnsw.cc:776-806) does. */
+ * the index byte-for-byte as Hnsw::SaveOptimizedIndex (hnsw.cc:776-806) or,
+ * for a regular index, SaveRegularIndexBin (hnsw.cc:810-841) does; save_data
+ * also writes the data points to filename + ".dat". */
 int nmsamd_save_index(nmsamd_index* h, const char* filename, int save_data);
 
 /* IndexWrapper::addDataPointBatch(data, ids) (nmslib.cc:342-344,
@@ -66,10 +71,11 @@ int nmsamd_add_data_point_batch(nmsamd_index* h, const float* data, int64_t n, i
  * saveIndex and for searching (it reaches the device on the first search).
  * n strings "key=value": M (16), efConstruction (200), maxM (M), maxM0 (2M),
  * mult (1/ln M), indexThreadQty (hardware threads), delaunay_type (2; 0, 1,
- * 2 supported), post (0, 1, 2), searchMethod (ignored); unknown keys raise
- * "Unknown parameters found!".  skip_optimized_index=1 and maxM0 above 64
- * are rejected (the device path searches optimized indices with lists of at
- * most 64).  With indexThreadQty=1 the graph equals the reference's: the
+ * 2 supported), post (0, 1, 2), skip_optimized_index (1 keeps the regular
+ * index, searched with the space's own distance and saved in the regular
+ * format), searchMethod (ignored); unknown keys raise "Unknown parameters
+ * found!".  maxM0 above 64 is rejected (the device searches lists of at most
+ * 64).  With indexThreadQty=1 the graph equals the reference's: the
  * level draws continue the calling thread's generator, seeded with 0 on each
  * thread's first use, as the reference's do.  Resets query-time params
  * (ef = 20).  print_progress is accepted and ignored. */
