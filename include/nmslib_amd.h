@@ -65,6 +65,18 @@ int nmsamd_save_index(nmsamd_index* h, const char* filename, int save_data);
 int nmsamd_add_data_point_batch(nmsamd_index* h, const float* data, int64_t n, int64_t dim,
                                 const int32_t* ids, int64_t* out_count);
 
+/* IndexWrapper::at(pos) -- __getitem__ (nmslib.cc:348, writeObject :308-320):
+ * copies up to cap floats of data point `pos` to out (out may be NULL) and
+ * its dimension to *dim.  A position outside the data fails with
+ * NMSAMD_EINVAL (the reference's std::out_of_range). */
+int nmsamd_get_data_point(nmsamd_index* h, int64_t pos, float* out, int64_t cap, int64_t* dim);
+
+/* IndexWrapper::getDistance(pos1, pos2) (nmslib.cc:350-353): the space's
+ * IndexTimeDistance between two data points -- for "l2" the square root of
+ * the squared distance, for "cosinesimil" 1 - cos of the raw vectors, in the
+ * reference's SSE summation order. */
+int nmsamd_get_distance(nmsamd_index* h, int64_t pos1, int64_t pos2, float* out);
+
 /* IndexWrapper::createIndex(index_params, print_progress) (nmslib.cc:89-96 ->
  * Hnsw::CreateIndex, hnsw.cc:183-470): builds the HNSW graph over the data
  * added so far on the host CPU and keeps it as the optimized index, ready for

@@ -57,6 +57,8 @@ SIGNATURES = {
     "nmsamd_save_index": (C.c_int, [_P, C.c_char_p, C.c_int]),
     "nmsamd_add_data_point_batch": (C.c_int, [_P, _P, _I64, _I64, _P, C.POINTER(C.c_int64)]),
     "nmsamd_create_index": (C.c_int, [_P, C.POINTER(C.c_char_p), C.c_int, C.c_int]),
+    "nmsamd_get_data_point": (C.c_int, [_P, _I64, _P, _I64, C.POINTER(C.c_int64)]),
+    "nmsamd_get_distance": (C.c_int, [_P, _I64, _I64, C.POINTER(C.c_float)]),
     "nmsamd_set_query_time_params": (C.c_int, [_P, C.POINTER(C.c_char_p), C.c_int]),
     "nmsamd_knn_query_batch": (C.c_int, [_P, _P, _I64, _I64, _I32, _I32, _P, _P, _P]),
     "nmsamd_knn_query_batch_device": (C.c_int, [_P, _P, _I64, _I32, _P, _P, _P, _P]),
@@ -333,6 +335,27 @@ class FloatIndex:
     def __len__(self):
         return int(load_library().nmsamd_size(self._h))
 
+    def __getitem__(self, pos):
+        """Data point `pos` as a list of floats (nmslib.cc:348, writeObject
+        :308-320); IndexError outside the data, like the reference's .at()."""
+        pos = int(pos)
+        if pos < 0 or pos >= len(self):
+            raise IndexError(f"data point position {pos} out of range")
+        dim = C.c_int64(0)
+        _check(load_library().nmsamd_get_data_point(self._h, pos, None, 0, C.byref(dim)))
+        out = np.empty(dim.value, np.float32)
+        _check(load_library().nmsamd_get_data_point(self._h, pos, _ptr(out), dim.value, None))
+        return [float(v) for v in out]
+
+    def getDistance(self, pos1, pos2):
+        """The space's distance between data points pos1 and pos2
+        (nmslib.cc:350-353)."""
+        if not (0 <= int(pos1) < len(self) and 0 <= int(pos2) < len(self)):
+            raise IndexError("data point position out of range")
+        out = C.c_float(0)
+        _check(load_library().nmsamd_get_distance(self._h, int(pos1), int(pos2), C.byref(out)))
+        return float(out.value)
+
     def __repr__(self):
         return f"<nmslib_amd.FloatIndex method='{self.method}' space='{self.space}'>"
 
@@ -347,3 +370,62 @@ def init(space: str = "cosinesimil", space_params=None, method: str = "hnsw",
     if DistType(dtype) != DistType.FLOAT:
         raise ValueError("only DistType.FLOAT is supported by the device query path")
     return FloatIndex(space, method, DataType(data_type), DistType(dtype))
+
+
+# -- the legacy module-level API (nmslib.cc:694-789) --------------------------
+def addDataPoint(index, id, datum):
+    return index.addDataPoint(id, datum)
+
+
+def addDataPointBatch(index, ids, data):
+    """Legacy form: numpy arrays only (ids int32, data float32); returns the
+    positions of the new points."""
+    if not (isinstance(ids, np.ndarray) and ids.dtype == np.int32):
+        raise ValueError("Invalid datatype for ids in addDataPointBatch")
+    if not (isinstance(data, np.ndarray) and data.dtype == np.float32):
+        raise ValueError("Invalid datatype for data in addDataPointBatch")
+    offset = len(index)
+    n = index.addDataPointBatch(data, ids)
+    return np.arange(offset, offset + n, dtype=np.int32)
+
+
+def setQueryTimeParams(index, params):
+    return index.setQueryTimeParams(params)
+
+
+def createIndex(index, index_params):
+    return index.createIndex(index_params)
+
+
+def saveIndex(index, filename):
+    return index.saveIndex(filename)
+
+
+def loadIndex(index, filename):
+    return index.loadIndex(filename)
+
+
+def knnQuery(index, k, query):
+    """Legacy form: the list of ids only."""
+    return [int(i) for i in index.knnQuery(query, k)[0]]
+
+
+def getDataPoint(index, pos):
+    return index[pos]
+
+
+def getDataPointQty(index):
+    return len(index)
+
+
+def getDistance(index, pos1, pos2):
+    return index.getDistance(pos1, pos2)
+
+
+def knnQueryBatch(index, num_threads, k, queries):
+    """Legacy form: a list of id lists."""
+    return [[int(i) for i in ids] for ids, _ in index.knnQueryBatch(queries, k, num_threads)]
+
+
+def freeIndex(index):
+    pass

@@ -500,6 +500,12 @@ class Graph {
 
 }  // namespace
 
+float space_distance(int space, const float* a, const float* b, size_t dim) {
+  const std::vector<const float*> vec{a, b};
+  BuildParams p;
+  return Graph(space, vec, dim, p).dist(0, 1);
+}
+
 void build_hnsw(int space, const std::vector<std::vector<char>>& objs, const BuildParams& p_in,
                 BuiltIndex& out) {
   if (space != kBuildL2 && space != kBuildCosine && space != kBuildNegDot && space != kBuildL1 &&

@@ -46,6 +46,10 @@ struct BuiltIndex {
   std::vector<int32_t> upper_off;   // start of a node's lists in `upper`, -1 when none
 };
 
+// Space::IndexTimeDistance(a, b) of two dense float vectors of `dim` floats
+// (the space's own distance, in the reference's SSE order).
+float space_distance(int space, const float* a, const float* b, size_t dim);
+
 // Builds from `objs`, the data points as the reference stores them (Object
 // buffers: i32 id, i32 label, u64 datalength, f32 data[dim]), in insertion
 // order.  Throws std::invalid_argument / std::runtime_error on bad input.

@@ -863,6 +863,42 @@ int nmsamd_add_data_point_batch(nmsamd_index* h, const float* data, int64_t n, i
   });
 }
 
+// Data point `pos` as a dense float vector (IndexWrapper::at, nmslib.cc:348).
+const std::vector<char>& data_point(const nmsamd_index* h, int64_t pos) {
+  if (pos < 0 || (uint64_t)pos >= h->data_objs.size())
+    fail(NMSAMD_EINVAL, "data point position " + std::to_string(pos) + " out of range");
+  const auto& o = h->data_objs[pos];
+  uint64_t dl = 0;
+  if (o.size() >= 16) std::memcpy(&dl, o.data() + 8, 8);
+  if (o.size() < 16 || dl + 16 != o.size() || dl % 4 != 0)
+    fail(NMSAMD_ERUNTIME, "data point " + std::to_string(pos) + " is not a dense float vector");
+  return o;
+}
+
+int nmsamd_get_data_point(nmsamd_index* h, int64_t pos, float* out, int64_t cap, int64_t* dim) {
+  return guarded([&] {
+    if (!h) fail(NMSAMD_EINVAL, "null handle");
+    std::lock_guard<std::mutex> g(h->mu);
+    const auto& o = data_point(h, pos);
+    const int64_t d = (int64_t)(o.size() - 16) / 4;
+    if (dim) *dim = d;
+    if (out) std::memcpy(out, o.data() + 16, 4 * (size_t)std::min(d, std::max<int64_t>(cap, 0)));
+  });
+}
+
+int nmsamd_get_distance(nmsamd_index* h, int64_t pos1, int64_t pos2, float* out) {
+  return guarded([&] {
+    if (!h || !out) fail(NMSAMD_EINVAL, "null argument");
+    std::lock_guard<std::mutex> g(h->mu);
+    const auto& a = data_point(h, pos1);
+    const auto& b = data_point(h, pos2);
+    if (a.size() != b.size()) fail(NMSAMD_ERUNTIME, "data points differ in dimension");
+    const int space = h->expected_func == 1 ? (int)kBuildL2 : h->expected_func;
+    *out = space_distance(space, reinterpret_cast<const float*>(a.data() + 16),
+                          reinterpret_cast<const float*>(b.data() + 16), (a.size() - 16) / 4);
+  });
+}
+
 int nmsamd_create_index(nmsamd_index* h, const char* const* kv, int nkv, int print_progress) {
   (void)print_progress;
   return guarded([&] {
