@@ -15,7 +15,7 @@ The cases cover what the fixtures idx_<name>.npz (written by make_goldens.py)
 do not: the other neighbour-selection rules (delaunay_type 0 and 1), both
 post-processing modes, L2 with a dimension that is not a multiple of 16
 (L2SqrExt), custom M / maxM / maxM0 / mult, and integer data whose distances
-tie often.
+tie often, 1 to 3 points, and data where every point appears four times.
 """
 from __future__ import annotations
 
@@ -47,7 +47,19 @@ CASES = {
     "b_custom_cos": ("cosinesimil", "decaying_normal", 1000, 24, 5,
                      "M=10,maxM=7,maxM0=15,mult=0.6,efConstruction=60"),
     "b_d1_nd": ("negdotprod", "gmm", 1000, 16, 6, "M=12,efConstruction=60,delaunay_type=1"),
+    "b_n1_l2": ("l2", "uniform", 1, 8, 71, "M=4,efConstruction=10"),
+    "b_n2_l2": ("l2", "uniform", 2, 8, 72, "M=4,efConstruction=10"),
+    "b_n3_cos": ("cosinesimil", "uniform", 3, 8, 73, "M=4,efConstruction=10"),
+    # every point four times over: all-zero distances between copies
+    "b_dup4_l2": ("l2", "uniform", 200, 8, 74, "M=6,efConstruction=20", 4),
 }
+
+
+def points(case):
+    """The case's data points: the generator's rows, each repeated `repeat` times."""
+    space, g, n, dim, seed, params = case[:6]
+    rep = case[6] if len(case) > 6 else 1
+    return np.repeat(getattr(D, g)(n // rep, dim, seed), rep, axis=0)
 
 
 def zero_unused_upper(pk):
@@ -61,10 +73,11 @@ def zero_unused_upper(pk):
 
 def main(only=None):
     meta = {}
-    for name, (space, g, n, dim, seed, params) in CASES.items():
+    for name, case in CASES.items():
+        space, g, n, dim, seed, params = case[:6]
         if only and name not in only:
             continue
-        x = getattr(D, g)(n, dim, seed)
+        x = points(case)
         with tempfile.TemporaryDirectory() as td:
             raw = os.path.join(td, "x.f32")
             x.astype(np.float32).tofile(raw)
@@ -73,7 +86,8 @@ def main(only=None):
                             params + ",indexThreadQty=1", path], check=True)
             pk = zero_unused_upper(indexpack.pack(path))
         indexpack.save_pack(pk, os.path.join(HERE, f"bld_{name}.npz"))
-        meta[name] = dict(space=space, generator=g, n=n, dim=dim, seed=seed, params=params)
+        meta[name] = dict(space=space, generator=g, n=n, dim=dim, seed=seed, params=params,
+                          repeat=case[6] if len(case) > 6 else 1)
         print(name, "done", flush=True)
     mpath = os.path.join(HERE, "build_cases.json")
     old = json.load(open(mpath)) if os.path.exists(mpath) else {}

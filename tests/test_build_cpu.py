@@ -77,7 +77,8 @@ def assert_same_graph(a, b):
 @pytest.mark.parametrize("name", sorted(BUILD_CASES))
 def test_build_is_byte_identical_to_reference(name):
     m = BUILD_CASES[name]
-    x = getattr(D, m["generator"])(m["n"], m["dim"], m["seed"])
+    rep = m.get("repeat", 1)
+    x = np.repeat(getattr(D, m["generator"])(m["n"] // rep, m["dim"], m["seed"]), rep, axis=0)
     ours = os.path.join(_TMP, f"{name}.ours")
     build_on_fresh_thread([(m["space"], x, m["params"], ours)])
     ref = os.path.join(_TMP, f"{name}.ref")
