@@ -82,8 +82,8 @@ int nmsamd_get_distance(nmsamd_index* h, int64_t pos1, int64_t pos2, float* out)
  * added so far on the host CPU and keeps it as the optimized index, ready for
  * saveIndex and for searching (it reaches the device on the first search).
  * n strings "key=value": M (16), efConstruction (200), maxM (M), maxM0 (2M),
- * mult (1/ln M), indexThreadQty (hardware threads), delaunay_type (2; 0, 1,
- * 2 supported), post (0, 1, 2), skip_optimized_index (1 keeps the regular
+ * mult (1/ln M), indexThreadQty (hardware threads), delaunay_type (2; 0-3,
+ * 3 with indexThreadQty=1 only), post (0, 1, 2), skip_optimized_index (1 keeps the regular
  * index, searched with the space's own distance and saved in the regular
  * format), searchMethod (ignored); unknown keys raise "Unknown parameters
  * found!".  maxM0 above 64 is rejected (the device searches lists of at most

@@ -3,7 +3,7 @@
 //   build_hnsw                  Hnsw::CreateIndex          src/method/hnsw.cc:183-470
 //   Graph::add                  Hnsw::add                  src/method/hnsw.cc:534-611
 //   Graph::search_level         kSearchElementsWithAttemptsLevel hnsw.cc:613-708
-//   Graph::select_h1 / _h2      getNeighborsByHeuristic1/2 include/method/hnsw.h:82-168
+//   Graph::select_h1 / _h2 / _h3 getNeighborsByHeuristic1/2/3 include/method/hnsw.h:82-258
 //   Graph::add_friend           HnswNode::addFriendlevel   include/method/hnsw.h:260-307
 //   random_level                Hnsw::getRandomLevel       include/method/hnsw.h:478-483
 //   dist_*                      the spaces' HiddenDistance (space_lp.cc:27-35,
@@ -245,6 +245,8 @@ class Graph {
         for (size_t cur : both) rs.push({dist((int32_t)cur, (int32_t)id), (int32_t)cur});
         if (p_.delaunay == 0) {
           while (rs.size() > p_.maxM0) rs.pop();
+        } else if (p_.delaunay == 3) {
+          select_h3((int32_t)id, rs, (int)p_.maxM0, 0);
         } else {
           select_h1(rs, (int)p_.maxM0);
         }
@@ -252,12 +254,16 @@ class Graph {
           out.push_back(rs.top().id);
           rs.pop();
         }
+        // heuristic 3 reads other lists: as in the reference's loop, a list
+        // is replaced as soon as it is made (single-threaded)
+        if (p_.delaunay == 3) nodes_[id].links[0].swap(out);
       } else {
         for (size_t cur : both) out.push_back((int32_t)cur);
       }
     };
     parallel(1, n_, body);
-    for (size_t id = 1; id < n_; ++id) nodes_[id].links[0].swap(rez[id]);
+    if (!(p_.post == 2 && p_.delaunay == 3))
+      for (size_t id = 1; id < n_; ++id) nodes_[id].links[0].swap(rez[id]);
     size_t mx = 0;
     for (size_t id = 1; id < n_; ++id) mx = std::max(mx, union_size[id]);
     return mx;
@@ -353,6 +359,8 @@ class Graph {
         while (rs.size() > p_.M) rs.pop();
       } else if (p_.delaunay == 1) {
         select_h1(rs, (int)p_.M);
+      } else if (p_.delaunay == 3) {
+        select_h3(id, rs, (int)p_.M, L);
       } else {
         select_h2(rs, (int)p_.M);
       }
@@ -452,6 +460,63 @@ class Graph {
     for (const Far& k : keep) rs.push({k.d, k.id});
   }
 
+  // Heuristic 3 (hnsw.h:169-258): widen the candidates with their own lists
+  // at level L, then take them nearest first into a high tier (no kept
+  // neighbour closer) or a second tier (only a passed-over candidate closer),
+  // keeping up to NN, high tier first.  The reference collects the widened set
+  // in a hash set of node pointers; its iteration order only changes the order
+  // among equal distances.  It reads other nodes' lists without their locks,
+  // so it runs single-threaded only.
+  void select_h3(int32_t self, NearHeap& rs, int nn, int L) {
+    std::unordered_set<int32_t> cands;
+    while (!rs.empty()) {
+      const int32_t t = rs.top().id;
+      cands.insert(t);
+      for (int32_t f : nodes_[t].links[L]) cands.insert(f);
+      rs.pop();
+    }
+    for (int32_t c : cands)
+      if (c != self) rs.push({dist(c, self), c});
+    if (rs.size() < (size_t)nn) return;
+    std::vector<Near> in(rs.size());
+    for (int i = (int)rs.size() - 1; i >= 0; --i) {
+      in[i] = rs.top();
+      rs.pop();
+    }
+    std::vector<Near> passed, second, high;
+    for (const Near& c : in) {
+      if (high.size() >= (size_t)nn) break;
+      int good = 2;
+      for (const Near& t : passed)
+        if (dist(t.id, c.id) < c.d) {
+          good = 1;
+          break;
+        }
+      for (const Near& t : high)
+        if (dist(t.id, c.id) < c.d) {
+          good = 0;
+          break;
+        }
+      if (good)
+        for (const Near& t : second)
+          if (dist(t.id, c.id) < c.d) {
+            good = 0;
+            break;
+          }
+      if (good == 2) high.push_back(c);
+      else if (good == 1) second.push_back(c);
+      else passed.push_back(c);
+    }
+    for (const Near& t : high) {
+      if (rs.size() >= (size_t)nn) break;
+      rs.push(t);
+    }
+    for (const Near& t : second) {
+      if (rs.size() >= (size_t)nn) break;
+      rs.push(t);
+    }
+  }
+
   // addFriendlevel: append `e` to x's list at level L; when the list is over
   // capacity, re-select all but one with the build's heuristic (the list is
   // then stored farthest first), or drop the farthest for delaunay_type 0.
@@ -468,6 +533,7 @@ class Graph {
       for (int32_t v : lst) rs.push({dist(x, v), v});
       const int nn = (int)rs.size() - 1;
       if (p_.delaunay == 1) select_h1(rs, nn);
+      else if (p_.delaunay == 3) select_h3(x, rs, nn, L);
       else select_h2(rs, nn);
       lst.clear();
       while (!rs.empty()) {
@@ -511,9 +577,11 @@ void build_hnsw(int space, const std::vector<std::vector<char>>& objs, const Bui
   if (space != kBuildL2 && space != kBuildCosine && space != kBuildNegDot && space != kBuildL1 &&
       space != kBuildLInf)
     throw std::invalid_argument("unknown space code " + std::to_string(space));
-  if (p_in.delaunay < 0 || p_in.delaunay > 2)
-    throw std::invalid_argument("delaunay_type " + std::to_string(p_in.delaunay) +
-                                " is not supported (0, 1 and 2 are)");
+  if (p_in.delaunay < 0 || p_in.delaunay > 3)
+    throw std::invalid_argument("delaunay_type should be 0, 1, 2 or 3");
+  if (p_in.delaunay == 3 && p_in.threads > 1)
+    throw std::invalid_argument("delaunay_type 3 needs indexThreadQty=1 (its selection reads other "
+                                "nodes' lists while they may change)");
   if (p_in.post < 0 || p_in.post > 2) throw std::invalid_argument("post should be 0, 1 or 2");
   if (p_in.M == 0 || p_in.maxM == 0 || p_in.maxM0 == 0 || p_in.efConstruction == 0)
     throw std::invalid_argument("M, maxM, maxM0 and efConstruction must be positive");

@@ -27,7 +27,7 @@ struct BuildParams {
   size_t efConstruction = 200;
   size_t threads = 1;     // indexThreadQty
   double mult = 0;        // level multiplier (default 1/ln M)
-  int delaunay = 2;       // neighbour selection: 0 closest, 1 heuristic 1, 2 heuristic 2
+  int delaunay = 2;       // neighbour selection: 0 closest, heuristic 1, 2 or 3
   int post = 0;           // 1, 2: merge with a graph built in reverse order
   bool optimized = true;  // false (skip_optimized_index=1): keep the regular form, whose
                           // vectors are not normalised and whose distances are the space's

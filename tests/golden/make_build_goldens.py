@@ -12,7 +12,7 @@ upper-level lists set to zero: the reference leaves those bytes
 uninitialised (hnsw.cc:455-465 mallocs the lists without clearing them).
 
 The cases cover what the fixtures idx_<name>.npz (written by make_goldens.py)
-do not: the other neighbour-selection rules (delaunay_type 0 and 1), both
+do not: the other neighbour-selection rules (delaunay_type 0, 1 and 3), both
 post-processing modes, L2 with a dimension that is not a multiple of 16
 (L2SqrExt), custom M / maxM / maxM0 / mult, and integer data whose distances
 tie often, 1 to 3 points, and data where every point appears four times.
@@ -50,6 +50,11 @@ CASES = {
     "b_n1_l2": ("l2", "uniform", 1, 8, 71, "M=4,efConstruction=10"),
     "b_n2_l2": ("l2", "uniform", 2, 8, 72, "M=4,efConstruction=10"),
     "b_n3_cos": ("cosinesimil", "uniform", 3, 8, 73, "M=4,efConstruction=10"),
+    # heuristic 3 on tie-free data (the reference's hash-set order only
+    # decides among equal distances)
+    "b_d3_l2": ("l2", "uniform", 800, 16, 75, "M=8,efConstruction=40,delaunay_type=3"),
+    "b_d3_post2_l2": ("l2", "uniform", 600, 16, 76, "M=8,efConstruction=40,delaunay_type=3,post=2"),
+    "b_d3_cos": ("cosinesimil", "decaying_normal", 800, 24, 77, "M=10,efConstruction=50,delaunay_type=3"),
     # every point four times over: all-zero distances between copies
     "b_dup4_l2": ("l2", "uniform", 200, 8, 74, "M=6,efConstruction=20", 4),
 }

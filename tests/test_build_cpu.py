@@ -171,7 +171,8 @@ def test_multithreaded_build_invariants_and_recall():
     ({"M": 8, "foo": 1}, RuntimeError, "Unknown parameters found!"),
     (["M=abc"], RuntimeError, "Failed to convert value 'abc' from type: m"),
     (["M"], RuntimeError, "Wrong format of an argument"),
-    ({"delaunay_type": 3}, ValueError, "delaunay_type 3"),
+    ({"delaunay_type": 3, "indexThreadQty": 2}, ValueError, "delaunay_type 3 needs indexThreadQty=1"),
+    ({"delaunay_type": 4}, ValueError, "delaunay_type should be"),
     ({"post": 3}, ValueError, "post"),
     ({"M": 40}, ValueError, "device limit"),
     ({"M": 1}, ValueError, "mult"),
