@@ -90,7 +90,7 @@ int nmsamd_get_distance(nmsamd_index* h, int64_t pos1, int64_t pos2, float* out)
  * 64).  With indexThreadQty=1 the graph equals the reference's: the
  * level draws continue the calling thread's generator, seeded with 0 on each
  * thread's first use, as the reference's do.  Resets query-time params
- * (ef = 20).  print_progress is accepted and ignored. */
+ * (ef = 20).  print_progress reports the points inserted to stderr every 5 %. */
 int nmsamd_create_index(nmsamd_index* h, const char* const* kv, int n, int print_progress);
 
 /* setQueryTimeParams (nmslib.cc:599-607 -> Hnsw::SetQueryTimeParams,

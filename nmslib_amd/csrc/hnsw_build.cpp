@@ -34,6 +34,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <exception>
 #include <limits>
@@ -222,9 +223,17 @@ class Graph {
     const int l0 = random_level(p_.mult);
     init_node(0, l0);
     top_.store(pack(l0, 0));
+    std::atomic<size_t> done(1);
+    const size_t step = std::max<size_t>(1, n_ / 20);
     auto body = [&](size_t pos, Visited& vis) {
       const int32_t id = (int32_t)(reverse ? n_ - pos : pos);
       add(id, vis);
+      if (p_.progress) {  // the reference's ProgressDisplay on stderr, every 5 %
+        const size_t d = done.fetch_add(1) + 1;
+        if (d % step == 0 || d == n_)
+          std::fprintf(stderr, "[createIndex] %s%zu / %zu points inserted\n",
+                       reverse ? "reverse pass: " : "", d, n_);
+      }
     };
     parallel(1, n_, body);
   }

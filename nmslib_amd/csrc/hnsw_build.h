@@ -29,6 +29,7 @@ struct BuildParams {
   double mult = 0;        // level multiplier (default 1/ln M)
   int delaunay = 2;       // neighbour selection: 0 closest, heuristic 1, 2 or 3
   int post = 0;           // 1, 2: merge with a graph built in reverse order
+  bool progress = false;  // print_progress: report inserted points to stderr
   bool optimized = true;  // false (skip_optimized_index=1): keep the regular form, whose
                           // vectors are not normalised and whose distances are the space's
 };

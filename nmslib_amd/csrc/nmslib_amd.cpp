@@ -900,7 +900,6 @@ int nmsamd_get_distance(nmsamd_index* h, int64_t pos1, int64_t pos2, float* out)
 }
 
 int nmsamd_create_index(nmsamd_index* h, const char* const* kv, int nkv, int print_progress) {
-  (void)print_progress;
   return guarded([&] {
     if (!h) fail(NMSAMD_EINVAL, "null handle");
     std::lock_guard<std::mutex> g(h->mu);
@@ -931,6 +930,7 @@ int nmsamd_create_index(nmsamd_index* h, const char* const* kv, int nkv, int pri
     if (!has_maxM0) p.maxM0 = p.M * 2;
     if (!has_mult) p.mult = 1 / std::log(1.0 * p.M);
     p.optimized = skip_optimized == 0;
+    p.progress = print_progress != 0;
     if (p.maxM0 > (size_t)kMaxM0 || p.maxM > (size_t)kMaxM0)
       fail(NMSAMD_EINVAL, "maxM0 " + std::to_string(p.maxM0) + " exceeds the device limit of " +
                               std::to_string(kMaxM0));
