@@ -291,6 +291,10 @@ def main():
                     help="concurrent batch lanes of the serving run")
     ap.add_argument("--serve-batch", type=int, default=1024,
                     help="most queries one lane launches at once")
+    ap.add_argument("--build-threads", type=int, default=0,
+                    help="build the index with this library's createIndex on N host threads "
+                         "(M=16, efConstruction=500, as the reference-built pack) instead of "
+                         "loading the reference's file; 0 = load the reference's file")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="only a 1 GiB copy and one search launch (for rocprofv3 --pmc runs)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "hnsw_search_kernel_halfnormal1m_traffic.md"),
@@ -307,15 +311,27 @@ def main():
     from nmslib_amd import datasets as D
 
     tmpdir = tempfile.mkdtemp(prefix="nmsamd_bench_")
-    path, _ = prepare_index(tmpdir)
     q = D.queries(WORKLOAD, args.nq)
     gt = np.load(GT)
     gids, gd = gt["ids"][:args.nq], gt["dists"][:args.nq]
 
     idx = nmslib.init(method="hnsw", space="l2")
-    t = time.time()
-    idx.loadIndex(path)
-    log(f"[bench] loadIndex {time.time() - t:.1f}s {idx.info()}")
+    build_s = None
+    if args.build_threads > 0:
+        # the graph built here on the host (createIndex), saved for the CPU leg
+        idx.addDataPointBatch(D.base(WORKLOAD))
+        t = time.time()
+        idx.createIndex({"M": 16, "efConstruction": 500, "indexThreadQty": args.build_threads},
+                        print_progress=True)
+        build_s = time.time() - t
+        log(f"[bench] createIndex {build_s:.1f}s on {args.build_threads} threads {idx.info()}")
+        path = os.path.join(tmpdir, "halfnormal1m_built.hnsw")
+        idx.saveIndex(path)
+    else:
+        path, _ = prepare_index(tmpdir)
+        t = time.time()
+        idx.loadIndex(path)
+        log(f"[bench] loadIndex {time.time() - t:.1f}s {idx.info()}")
     dev = torch.device("cuda:0")
 
     if args.pmc_pass:
@@ -389,7 +405,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16" if head["form"].startswith("bf16") else "f32",
-        "data": "synthetic (seeded generator; reference-built index)",
+        "data": ("synthetic (seeded generator; index built by createIndex in "
+                 f"{build_s:.0f} s on {args.build_threads} host threads)" if build_s is not None
+                 else "synthetic (seeded generator; reference-built index)"),
         "config": {"workload": "1,000,000 x 128 l2 half-normal u8-valued, M=16 efC=500, "
                                "10,000 queries, k=10",
                    "headline_form": head["form"], "efSearch": head["efSearch"],
