@@ -1288,6 +1288,9 @@ int nmsamd_server_open(nmsamd_index* h, int lanes, int max_queries, int max_ef, 
       fail(NMSAMD_EINVAL, "lanes in 1..16 and positive max_queries / max_ef / max_k required");
     if (max_ef >= 1000 && h->algo != kV1Merge)
       fail(NMSAMD_EINVAL, "the server runs V1Merge: keep ef below 1000 or set algoType=v1merge");
+    if (h->regular && h->storage != kF32)
+      fail(NMSAMD_EINVAL, "reduced-precision traversal needs an optimized index (a regular one is "
+                          "searched with the space's own FP32 distance)");
     auto s = std::make_unique<nmsamd_server>();
     s->h = h;
     s->max_queries = max_queries;

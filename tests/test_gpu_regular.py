@@ -82,3 +82,39 @@ def test_regular_index_rejects_reduced_precision():
     idx.setPrecision("bf16", 2)
     with pytest.raises(ValueError, match="needs an optimized index"):
         idx.knnQueryBatchArrays(R.fixture(name)["queries"], 10)
+
+
+@pytest.mark.parametrize("beam", [0, 4])
+def test_regular_index_serves_like_the_batch_entry_point(beam):
+    """The serving lanes over a regular index: a request's answers equal the
+    plain batch entry point's (exact mode, and beam mode, which uses the same
+    kernels with the space's distance)."""
+    require_gpu()
+    from nmslib_amd import serving
+    name = "r_u128_l2"
+    path = os.path.join(_TMP, "serve.reg")
+    R.write_index(name, path)
+    idx = nm.init(method="hnsw", space="l2")
+    idx.addDataPointBatch(R.base_vectors(name))
+    idx.loadIndex(path)
+    idx.setBeam(beam)
+    q = R.fixture(name)["queries"][:40]
+    ks = np.full(len(q), 10, np.int32)
+    efs = np.full(len(q), 64, np.int32)
+    srv = serving.Server(idx, lanes=1, max_queries=64, max_ef=64, max_k=10, use_graph=False)
+    srv.submit(0, q, ks, efs)
+    si, sd, sc = srv.fetch(0)
+    srv.close()
+    ids, dists, counts = idx.knnQueryBatchUneven(q, ks, efs)
+    np.testing.assert_array_equal(sc, counts)
+    np.testing.assert_array_equal(si, ids)
+    np.testing.assert_array_equal(sd.view(np.uint32), dists.view(np.uint32))
+    if beam == 0:  # exact mode: the reference's answers
+        fx = R.fixture(name)
+        ci, cd = F.canon(ids[:, :10], dists[:, :10])
+        idx.setQueryTimeParams({"efSearch": 64, "algoType": "v1merge"})
+        ri, rd, rc = idx.knnQueryBatchArrays(q, 10)
+        np.testing.assert_array_equal(ci, F.canon(ri, rd)[0])
+    idx.setPrecision("bf16", 2)
+    with pytest.raises(ValueError, match="needs an optimized index"):
+        serving.Server(idx, lanes=1, max_queries=64, max_ef=64, max_k=10, use_graph=False)
