@@ -290,7 +290,13 @@ class FloatIndex:
         """knnQueryBatch (nmslib.cc:142-184): a list of (ids int32, dists float32)
         per query, ascending.  num_threads is accepted and ignored."""
         ids, dists, counts = self.knnQueryBatchArrays(queries, k)
-        return [(ids[i, :c].copy(), dists[i, :c].copy()) for i, c in enumerate(counts)]
+        # one row view per query of the result arrays (a per-row copy costs
+        # about four times as much); rows with fewer than k results are cut
+        rows_i, rows_d = list(ids), list(dists)
+        for i in np.nonzero(counts < ids.shape[1])[0]:
+            rows_i[i] = rows_i[i][:counts[i]]
+            rows_d[i] = rows_d[i][:counts[i]]
+        return list(zip(rows_i, rows_d))
 
     def knnQuery(self, vector, k: int = 10):
         ids, dists = self.knnQueryBatch(np.asarray(vector, np.float32).reshape(1, -1), k)[0]

@@ -103,6 +103,12 @@ def test_gpu_edge_cases():
     assert np.all(counts == 60)
     for row in ids:
         assert sorted(row[:60].tolist()) == list(range(60))
+    # the list form holds only the found results (nmslib.cc:170-184)
+    rows = idx.knnQueryBatch(q[:3], 100)
+    for (ri, rd), i, d in zip(rows, ids, dists):
+        assert ri.dtype == np.int32 and rd.dtype == np.float32 and len(ri) == len(rd) == 60
+        np.testing.assert_array_equal(ri, i[:60])
+        np.testing.assert_array_equal(rd, d[:60])
     idx.setQueryTimeParams({"efSearch": 10})
     # k = 0 and an empty batch
     ids, dists, counts = idx.knnQueryBatchArrays(q[:3], 0)
