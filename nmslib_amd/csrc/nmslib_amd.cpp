@@ -111,6 +111,7 @@ struct nmsamd_index {
   // file header (hnsw.cc:779-788)
   bool loaded = false;
   bool on_device = false;  // uploaded on the first search when no device was visible before
+  int servers = 0;         // open serving handles: they hold this index's device pointers
   // regular (non-optimized) index (hnsw.cc:810-841, 943-991): searched with the
   // space's own distance; saved back in its own format
   bool regular = false;
@@ -714,6 +715,13 @@ double to_double(const std::string& v) {
   return out;
 }
 
+// Refuses to replace or free an index that open servers still search.
+void require_no_servers(const nmsamd_index* h) {
+  if (h->servers > 0)
+    fail(NMSAMD_EINVAL, "close the servers opened on this index first (" +
+                            std::to_string(h->servers) + " open)");
+}
+
 template <class F>
 int guarded(F&& f) {
   try {
@@ -759,13 +767,17 @@ int nmsamd_init(const char* space, const char* method, nmsamd_index** out) {
 }
 
 int nmsamd_free(nmsamd_index* h) {
-  return guarded([&] { delete h; });
+  return guarded([&] {
+    if (h) require_no_servers(h);
+    delete h;
+  });
 }
 
 int nmsamd_load_index(nmsamd_index* h, const char* filename, int load_data) {
   return guarded([&] {
     if (!h || !filename) fail(NMSAMD_EINVAL, "null argument");
     std::lock_guard<std::mutex> g(h->mu);
+    require_no_servers(h);
     h->loaded = false;
     if (load_data) {  // without load_data the points added so far stay (nmslib.cc:103-107)
       h->data_objs.clear();
@@ -903,6 +915,7 @@ int nmsamd_create_index(nmsamd_index* h, const char* const* kv, int nkv, int pri
   return guarded([&] {
     if (!h) fail(NMSAMD_EINVAL, "null handle");
     std::lock_guard<std::mutex> g(h->mu);
+    require_no_servers(h);
     // Hnsw::CreateIndex parameters and defaults (hnsw.cc:187-205)
     BuildParams p;
     bool has_maxM = false, has_maxM0 = false, has_mult = false;
@@ -1283,6 +1296,7 @@ int nmsamd_server_open(nmsamd_index* h, int lanes, int max_queries, int max_ef, 
                        int use_graph, nmsamd_server** out) {
   return guarded([&] {
     if (!h || !out) fail(NMSAMD_EINVAL, "null argument");
+    std::lock_guard<std::mutex> g(h->mu);  // against a concurrent load or build
     require_index(h);
     if (lanes < 1 || lanes > 16 || max_queries < 1 || max_ef < 1 || max_k < 1)
       fail(NMSAMD_EINVAL, "lanes in 1..16 and positive max_queries / max_ef / max_k required");
@@ -1348,6 +1362,7 @@ int nmsamd_server_open(nmsamd_index* h, int lanes, int max_queries, int max_ef, 
         hip_check(hipGraphInstantiate(&L.exec, L.graph, nullptr, nullptr, 0), "graph instantiate");
       }
     }
+    ++h->servers;
     *out = s.release();
   });
 }
@@ -1428,6 +1443,10 @@ int nmsamd_server_close(nmsamd_server* s) {
     if (!s) return;
     for (auto& L : s->lanes)
       if (L.stream) (void)hipStreamSynchronize(L.stream);
+    if (s->h) {
+      std::lock_guard<std::mutex> g(s->h->mu);
+      --s->h->servers;
+    }
     delete s;
   });
 }

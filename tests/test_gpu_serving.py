@@ -73,3 +73,19 @@ def test_autotune_picks_a_form_within_the_recall_slack():
     # the choice is applied: a second call hits the cache
     again = autotune.autotune(idx, q, ks, efs)
     assert again["form"] == best["form"]
+
+
+def test_index_cannot_be_replaced_under_an_open_server():
+    # a server holds the index's device pointers (in captured graphs too):
+    # loading or building another index on the handle waits for its close
+    name = "u128_l2"
+    idx = load(name, F.index_path(name), "l2")
+    srv = serving.Server(idx, lanes=1, max_queries=8, max_ef=64, max_k=10, use_graph=True)
+    with pytest.raises(ValueError, match="close the servers"):
+        idx.loadIndex(F.index_path(name))
+    idx.addDataPointBatch(F.base_vectors(name)[:100])
+    with pytest.raises(ValueError, match="close the servers"):
+        idx.createIndex({"M": 8})
+    srv.close()
+    idx.loadIndex(F.index_path(name))  # allowed again
+    assert idx.info()["n"] == F.META[name]["n"]
