@@ -198,7 +198,9 @@ int nmsamd_brute_force(nmsamd_index* h, const float* q, int64_t nq, int64_t dim,
  * max_k, 0 <= ef <= max_ef) and runs V1Merge (or the index's beam mode /
  * precision settings at open time).  use_graph captures each lane's launch
  * as a hipGraph once and replays it per batch.  The index handle must not be
- * reconfigured while a server is open on it. */
+ * reconfigured while a server is open on it: nmsamd_load_index,
+ * nmsamd_create_index and nmsamd_free fail with NMSAMD_EINVAL until its
+ * servers are closed. */
 typedef struct nmsamd_server nmsamd_server;
 
 int nmsamd_server_open(nmsamd_index* h, int lanes, int max_queries, int max_ef, int max_k,
