@@ -349,6 +349,7 @@ class Graph {
         while (changed) {
           changed = false;
           copy_links(cur, L, buf);  // the scan continues over this list if `cur` moves
+          for (int32_t nb : buf) __builtin_prefetch(vec_[nb]);
           for (int32_t nb : buf) {
             const float d = dist(id, nb);
             if (d < cur_d) {
@@ -401,6 +402,10 @@ class Graph {
       if (cur.d > rs.top().d) break;
       copy_links(cur.id, L, buf);
       cand.pop();
+      for (int32_t nb : buf) {  // start the vector and mark loads (hnsw.cc:658-662)
+        __builtin_prefetch(vec_[nb]);
+        __builtin_prefetch(&vis.mark[nb]);
+      }
       for (int32_t nb : buf) {
         if (vis.mark[nb] == vis.epoch) continue;
         vis.mark[nb] = vis.epoch;
