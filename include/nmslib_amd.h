@@ -110,7 +110,10 @@ int nmsamd_knn_query_batch(nmsamd_index* h, const float* q, int64_t nq, int64_t 
 
 /* Same over device memory already resident in HBM; launches on `stream`
  * (a hipStream_t, NULL = the library's own stream) and returns without
- * waiting for completion. */
+ * waiting for completion.  The handle's launches share its work counter,
+ * visited marks and events, so a later search or brute force of the same
+ * handle on another stream waits on the device for this one to finish
+ * (hipStreamWaitEvent); launches on one stream are ordered by the stream. */
 int nmsamd_knn_query_batch_device(nmsamd_index* h, const float* d_q, int64_t nq, int32_t k,
                                   int32_t* d_ids, float* d_dists, int32_t* d_counts,
                                   void* stream);

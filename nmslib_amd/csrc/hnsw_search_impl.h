@@ -50,6 +50,16 @@ namespace nmsamd {
 
 __device__ __forceinline__ float kInf() { return __builtin_inff(); }
 
+// A node id read back from the LDS search state just before it addresses
+// global memory.  An id outside [0, n) could only come from a corrupted
+// state: it is reported through the error word (bit 2, raised as an error by
+// the host) and replaced by node 0, so no access leaves the index.
+__device__ __forceinline__ int checked_node(int id, int n, int* err) {
+  if ((unsigned)id < (unsigned)n) return id;
+  if (err) atomicOr(err, 2);
+  return 0;
+}
+
 template <int FUNC>
 __device__ __forceinline__ float term(float q, float x) {
   if constexpr (FUNC == kL2Sqr16 || FUNC == kL2Sqr) {
@@ -624,11 +634,12 @@ struct PathNV {
 };
 
 // cdist[s] = distance of cand[s] for s < ncand (fallback: a valid node id used
-// by idle lane pairs so every shuffle sees all lanes)
+// by idle lane pairs so every shuffle sees all lanes).  The ids come back from
+// LDS, so each passes checked_node before it addresses a vector.
 template <int D, int FUNC, int ST, int WIDE>
 __device__ __forceinline__ void dist_cands(const float* __restrict__ qs, const DevIndex& ix,
                                            const int* cand, float* cdist, int ncand, int lane,
-                                           int h, int fallback) {
+                                           int h, int fallback, int* err) {
   constexpr int NV = PathNV<ST, WIDE>::value;
   if constexpr (NV > 0 && D > 0 && D % 4 == 0 && (ST != kF32 || FUNC <= kNegDot)) {
     for (int b = 0; b < ncand; b += 32 * NV) {
@@ -636,7 +647,7 @@ __device__ __forceinline__ void dist_cands(const float* __restrict__ qs, const D
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         const int s = b + 32 * v + (lane >> 1);
-        const int id = s < ncand ? cand[s] : fallback;
+        const int id = checked_node(s < ncand ? cand[s] : fallback, (int)ix.n, err);
         vec_load<D, ST>(r[v], ix, id, h);
       }
 #pragma unroll
@@ -650,7 +661,7 @@ __device__ __forceinline__ void dist_cands(const float* __restrict__ qs, const D
     for (int b = 0; b < ncand; b += 32) {
       const int s = b + (lane >> 1);
       const bool ok = s < ncand;
-      const int id = ok ? cand[s] : fallback;
+      const int id = checked_node(ok ? cand[s] : fallback, (int)ix.n, err);
       const float dd = dist_node<D, FUNC, ST>(qs, ix, id, h);
       if (ok && h == 0) cdist[s] = dd;
     }
@@ -805,7 +816,9 @@ __device__ __forceinline__ void greedy_descent(const DevIndex& ix, const float* 
   for (int lev = ix.maxlevel; lev > 0; --lev) {
     for (;;) {
       const int off = ix.up_off[cur];
-      if (off < 0) break;  // a malformed index cannot send the scan out of bounds
+      // cur has a list at level lev: the entry point reaches maxlevel and the
+      // loader checks that every level-L neighbour has L levels of its own
+      if (off < 0) break;
       const int32_t* lst = ix.up + off + (ix.maxM + 1) * (lev - 1);
       const int cnt = min(lst[0], ix.maxM);
       nd += cnt;
@@ -845,16 +858,6 @@ __device__ __forceinline__ void greedy_descent(const DevIndex& ix, const float* 
 
 // position of the first item at or after p that is not expanded yet (n if
 // none): one chunk of 64, then four chunks per round trip
-// A node id read back from the LDS search state just before it addresses
-// global memory.  An id outside [0, n) could only come from a corrupted
-// state: it is reported through the error word (bit 2, raised as an error by
-// the host) and replaced by node 0, so no access leaves the index.
-__device__ __forceinline__ int checked_node(int id, int n, int* err) {
-  if ((unsigned)id < (unsigned)n) return id;
-  if (err) atomicOr(err, 2);
-  return 0;
-}
-
 __device__ __forceinline__ int first_unused(const uint2* A, int p, int n, int lane) {
   if (p < n) {
     const int i = p + lane;
@@ -1823,7 +1826,7 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
       h0 = max(0, ncand - P1 * r);
       if (h0 < ncand) link.post(lane, kCmdDist, ncand, h0, __float_as_int(topk), nsnap, ef, cur);
     }
-    dist_cands<D, FUNC, ST, DPATH>(w.qs, ix, w.cand, w.cdist, h0, lane, h, cur);
+    dist_cands<D, FUNC, ST, DPATH>(w.qs, ix, w.cand, w.cdist, h0, lane, h, cur, a.err);
     WSYNC();
     lap(2);
     // keep what the snapshot admits; order does not matter (sorted next)
@@ -1949,7 +1952,8 @@ __device__ __forceinline__ void beam_helper(const SearchArgs& a, const WaveLds& 
       const int nsnap = __builtin_amdgcn_readfirstlane(c1.x);
       const int ef = __builtin_amdgcn_readfirstlane(c1.y);
       const int fb = __builtin_amdgcn_readfirstlane(c1.z);
-      dist_cands<D, FUNC, ST, 1>(w.qs, a.ix, w.cand + h0, w.cdist + h0, ncand - h0, lane, h, fb);
+      dist_cands<D, FUNC, ST, 1>(w.qs, a.ix, w.cand + h0, w.cdist + h0, ncand - h0, lane, h, fb,
+                                 a.err);
       WSYNC();
       const int m1 = admit(w, h0, ncand, topk, nsnap, ef, lane, ltm, w.items + h0);
       if (lane == 0) w.ctl[16] = m1;

@@ -154,6 +154,7 @@ struct nmsamd_index {
   DevBuf<float> d_xnorm, d_bq, d_bpd;  // brute force: |x|^2, padded queries, partial lists
   DevBuf<int32_t> d_bpi;
   bool pending = false;
+  hipStream_t last_stream = nullptr;  // stream of the last search launch (ev1 follows it)
   int64_t last_nq = 0;
   SearchStats last_stats{};
   int64_t last_launch[5] = {0, 0, 0, 0, 0};  // kernel mode, blocks, waves/block, LDS/wave, entries
@@ -413,15 +414,26 @@ void check_index(nmsamd_index* h) {
     std::sort(tmp.begin(), tmp.end());
     if (std::adjacent_find(tmp.begin(), tmp.end()) != tmp.end()) dups = true;
   }
+  // a node's level is the number of its upper-level blocks
+  auto level_of = [&](int64_t i) {
+    return h->upper_off[i] < 0 ? 0 : (int)(h->sizemass[i] / (4 * (h->maxM + 1)));
+  };
   for (size_t i = 0; i < h->upper_off.size(); ++i) {
     if (h->upper_off[i] < 0) continue;
-    const int levels = (int)(h->sizemass[i] / (4 * (h->maxM + 1)));
+    const int levels = level_of((int64_t)i);
     for (int L = 0; L < levels; ++L) {
       const int32_t* lst = h->upper.data() + h->upper_off[i] + (h->maxM + 1) * L;
       if (lst[0] < 0 || (uint64_t)lst[0] > h->maxM)
         fail(NMSAMD_ERUNTIME, "corrupt upper-level list count");
-      for (int j = 1; j <= lst[0]; ++j)
+      for (int j = 1; j <= lst[0]; ++j) {
         if (lst[j] < 0 || lst[j] >= n) fail(NMSAMD_ERUNTIME, "upper-level neighbour id out of range");
+        // the greedy descent reads a level-(L+1) neighbour's own list at that
+        // level: HNSW links a node only to nodes present at the link's level
+        if (level_of(lst[j]) < L + 1)
+          fail(NMSAMD_ERUNTIME, "upper-level neighbour " + std::to_string(lst[j]) + " of node " +
+                                    std::to_string(i) + " has no list at level " +
+                                    std::to_string(L + 1));
+      }
     }
   }
   if (n > 0 && h->maxlevel > 0) {
@@ -491,18 +503,25 @@ void ensure_slots(nmsamd_index* h, int slots) {
   h->slots = slots;
 }
 
+// The reduced-precision copy the current storage traverses, converted on the
+// first use.  The conversion finishes before this returns: searches and
+// serving lanes on other streams read the copy right after.
 void ensure_lowp(nmsamd_index* h) {
+  bool converted = false;
   if (h->storage == kBF16 && !h->d_vec16.p) {
     h->d_vec16.ensure((size_t)std::max<int64_t>(h->n, 1) * h->dstride);
     hip_check(launch_convert_lowp(h->d_vec.p, h->d_vec16.p, nullptr, nullptr, h->n, h->dim,
                                   h->dstride, kBF16, h->stream), "convert bf16");
+    converted = true;
   }
   if (h->storage == kFP8 && !h->d_vec8.p) {
     h->d_vec8.ensure((size_t)std::max<int64_t>(h->n, 1) * h->dstride);
     h->d_scale8.ensure((size_t)std::max<int64_t>(h->n, 1));
     hip_check(launch_convert_lowp(h->d_vec.p, nullptr, h->d_vec8.p, h->d_scale8.p, h->n, h->dim,
                                   h->dstride, kFP8, h->stream), "convert fp8");
+    converted = true;
   }
+  if (converted) hip_check(hipStreamSynchronize(h->stream), "reduced-precision conversion");
 }
 
 DevIndex dev_index(nmsamd_index* h) {
@@ -551,6 +570,16 @@ int lds_wave_bytes(int cap, int dstride, int ent, bool beam) {
   return (b + 15) / 16 * 16;
 }
 
+// The handle's launches share its work counter, error word, visited slots,
+// epochs and events: a launch on another stream than the last one starts only
+// after the last has finished (its end is ev1; the device entry point returns
+// without waiting).  The caller records ev1 after its own launch.
+void follow_last_launch(nmsamd_index* h, hipStream_t s) {
+  if (h->last_stream && h->last_stream != s)
+    hip_check(hipStreamWaitEvent(s, h->ev1, 0), "hipStreamWaitEvent");
+  h->last_stream = s;
+}
+
 // Launches the search of nq queries whose inputs/outputs are device pointers;
 // k_of / ef_of are optional device arrays whose maxima are max_k / max_ef.
 void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, const int32_t* k_of,
@@ -588,12 +617,15 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   // Two waves per query run at most three queries per CU, the shape they are
   // measured and tested in (variants at four queries per CU faulted the GPU
   // on a 10,000-point index with the cause not found), and are chosen
-  // automatically only where LDS already limits a CU to three queries.
+  // automatically only where LDS already limits a CU to three queries.  The
+  // limit is the hardware's, not only the grid's: a two-wave block reserves
+  // more than a quarter of the CU's LDS, so a fourth block never fits.
   const size_t fit = kLdsPerCu / lds;
   const bool pair = beam && (h->beam_waves == 2 || (h->beam_waves == 0 && fit <= 3));
+  const int lds_launch = pair ? std::max(lds, (int)(kLdsPerCu / 4) + 16) : lds;
   const int wpb = pair ? 1 : (int)std::max<size_t>(1, std::min<size_t>(kWavesPerBlock, fit));
   const int per_cu = (int)std::max<size_t>(
-      1, std::min<size_t>(pair ? 3 : 8, kLdsPerCu / ((size_t)lds * wpb)));
+      1, std::min<size_t>(pair ? 3 : 8, kLdsPerCu / ((size_t)lds_launch * wpb)));
   const int mode = !beam ? kModeAll
                          : (pair ? kModeBeamPair : (wpb * per_cu <= 4 ? kModeBeamWide : kModeBeam));
   int64_t max_blocks = (int64_t)h->cu_count * per_cu;
@@ -610,6 +642,7 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
     h->d_oldk.ensure((size_t)slots * heap_cap);
     h->d_oldi.ensure((size_t)slots * heap_cap);
   }
+  follow_last_launch(h, s);
   h->d_work.ensure(2);
   hip_check(launch_fill_i32(h->d_work.p, 0, 2, s), "fill");
   if (h->collect) {
@@ -647,12 +680,12 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   a.old_heap_id = h->d_oldi.p;
   a.old_heap_cap = heap_cap;
   hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
-  hip_check(launch_search(a, blocks, wpb, lds, mode, s), "search kernel launch");
+  hip_check(launch_search(a, blocks, wpb, lds_launch, mode, s), "search kernel launch");
   hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
   h->last_launch[0] = mode;
   h->last_launch[1] = blocks;
   h->last_launch[2] = wpb;
-  h->last_launch[3] = lds;
+  h->last_launch[3] = lds_launch;
   h->last_launch[4] = ent;
   h->pending = true;
 }
@@ -1232,6 +1265,11 @@ struct ServerLane {
 struct nmsamd_server {
   nmsamd_index* h = nullptr;
   int max_queries = 0, max_ef = 0, max_k = 0, cap = 0, ent = 0, lds = 0, wpb = 0, wide = 0;
+  // the search form and device index as they were at open: the LDS layout,
+  // the captured graphs and every later launch follow these, whatever the
+  // index's setters change afterwards
+  int storage = kF32, rerank = 0, beam = 0;
+  DevIndex ix{};
   int slots_per_lane = 0;
   bool use_graph = false;
   DevBuf<uint8_t> visited, epoch;
@@ -1253,12 +1291,8 @@ struct nmsamd_server {
 namespace {
 
 SearchArgs server_args(nmsamd_server* s, ServerLane& L) {
-  nmsamd_index* h = s->h;
   SearchArgs a{};
-  a.ix = dev_index(h);
-  a.ix.visited = s->visited.p;
-  a.ix.epoch = s->epoch.p;
-  a.ix.vstride = h->vstride_server;
+  a.ix = s->ix;
   a.queries = L.q.p;
   a.nq = 0;
   a.nq_dev = L.nq_dev.p;
@@ -1271,9 +1305,9 @@ SearchArgs server_args(nmsamd_server* s, ServerLane& L) {
   a.out_counts = L.counts.p;
   a.work = L.work.p;
   a.err = L.work.p + 1;
-  a.storage = h->storage;
-  a.rerank = (h->storage != kF32 && h->rerank_mult > 0) ? h->rerank_mult : 0;
-  a.beam = h->beam;
+  a.storage = s->storage;
+  a.rerank = s->rerank;
+  a.beam = s->beam;
   a.ent = s->ent;
   a.algo = kAlgoV1Merge;
   a.slot_base = L.slot_base;
@@ -1317,7 +1351,10 @@ int nmsamd_server_open(nmsamd_index* h, int lanes, int max_queries, int max_ef, 
     s->ent = step_entries(h);
     s->lds = lds_wave_bytes(s->cap, h->dstride, s->ent, h->beam > 0);
     if ((size_t)s->lds > kLdsPerCu) fail(NMSAMD_EINVAL, "max_ef / max_k exceed the LDS of one wave");
-    ensure_lowp(h);
+    ensure_lowp(h);  // finished on return: the lanes' streams read the copy at once
+    s->storage = h->storage;
+    s->rerank = (h->storage != kF32 && h->rerank_mult > 0) ? h->rerank_mult : 0;
+    s->beam = h->beam;
     s->wpb = (int)std::max<size_t>(1, std::min<size_t>(kWavesPerBlock, kLdsPerCu / s->lds));
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, kLdsPerCu / ((size_t)s->lds * s->wpb)));
     s->wide = (h->beam > 0 && s->wpb * per_cu <= 4) ? 1 : 0;
@@ -1331,6 +1368,10 @@ int nmsamd_server_open(nmsamd_index* h, int lanes, int max_queries, int max_ef, 
     s->epoch.ensure((size_t)slots);
     hip_check(hipMemset(s->visited.p, 0, (size_t)slots * h->vstride_server), "memset");
     hip_check(hipMemset(s->epoch.p, 0, (size_t)slots), "memset");
+    s->ix = dev_index(h);
+    s->ix.visited = s->visited.p;
+    s->ix.epoch = s->epoch.p;
+    s->ix.vstride = h->vstride_server;
     s->use_graph = use_graph != 0;
     for (int i = 0; i < lanes; ++i) s->lanes.emplace_back();
     for (int i = 0; i < lanes; ++i) {
@@ -1507,6 +1548,7 @@ extern "C" int nmsamd_brute_force(nmsamd_index* h, const float* q, int64_t nq, i
     h->d_ids.ensure((size_t)nq * k);
     h->d_dists.ensure((size_t)nq * k);
     h->d_counts.ensure((size_t)nq);
+    follow_last_launch(h, s);
     hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
     hip_check(launch_brute(h->d_vec.p, h->d_xnorm.p, h->d_bq.p, h->n, nq, h->dstride, h->func, k,
                            nchunks, h->d_bpd.p, h->d_bpi.p, h->d_ext.p, h->d_ids.p, h->d_dists.p,

@@ -89,3 +89,48 @@ def test_index_cannot_be_replaced_under_an_open_server():
     srv.close()
     idx.loadIndex(F.index_path(name))  # allowed again
     assert idx.info()["n"] == F.META[name]["n"]
+
+
+def test_server_keeps_its_form_when_the_index_setters_change():
+    # a server searches with the precision, rerank and beam it was opened
+    # with (its LDS layout and captured graphs follow them); changing them on
+    # the index afterwards affects the batch entry points only
+    name = "u128_l2"
+    idx = load(name, F.index_path(name), "l2")
+    pool = F.results(name)["queries"][:64]
+    ks = np.full(len(pool), 10, np.int32)
+    efs = np.full(len(pool), 100, np.int32)
+    alone = idx.knnQueryBatchUneven(pool, ks, efs)
+    for use_graph in (False, True):
+        srv = serving.Server(idx, lanes=1, max_queries=64, max_ef=128, max_k=10,
+                             use_graph=use_graph)
+        idx.setPrecision("bf16", 2)
+        idx.setBeam(16)
+        srv.submit(0, pool, ks, efs)
+        ids, dists, counts = srv.fetch(0)
+        idx.setPrecision("fp32", 0)
+        idx.setBeam(0)
+        srv.close()
+        np.testing.assert_array_equal(counts, alone[2])
+        np.testing.assert_array_equal(ids, alone[0][:, :10])
+        np.testing.assert_array_equal(dists, alone[1][:, :10])
+
+
+def test_bf16_server_opened_before_any_batch_search():
+    # the server is the first user of the BF16 copy: its conversion finishes
+    # before the lanes' first batch reads it
+    name = "hn128_l2"  # integer coordinates: BF16 traversal equals FP32
+    idx = load(name, F.index_path(name), "l2")
+    idx.setPrecision("bf16", 0)
+    srv = serving.Server(idx, lanes=2, max_queries=200, max_ef=100, max_k=10, use_graph=True)
+    pool = F.results(name)["queries"]
+    ks = np.full(len(pool), 10, np.int32)
+    efs = np.full(len(pool), 100, np.int32)
+    srv.submit(1, pool, ks, efs)
+    ids, dists, counts = srv.fetch(1)
+    srv.close()
+    idx.setPrecision("fp32", 0)
+    ref = idx.knnQueryBatchUneven(pool, ks, efs)
+    np.testing.assert_array_equal(counts, ref[2])
+    np.testing.assert_array_equal(ids, ref[0][:, :10])
+    np.testing.assert_array_equal(dists, ref[1][:, :10])

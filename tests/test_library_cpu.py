@@ -96,3 +96,41 @@ def test_extension_setters_validate_their_arguments():
     idx.setPrecision("fp32", 0)
     with pytest.raises(ValueError):
         idx.setPrecision("bf16", -1)
+
+
+def test_load_rejects_upper_neighbour_without_that_level(tmp_path):
+    # the greedy descent reads a level-L neighbour's own list at level L
+    # (hnsw_distfunc_opt.cc:168-198); a valid HNSW links a node at level L
+    # only to nodes that reach level L, so a file that breaks this is refused
+    # at load instead of sending the device scan past the end of the lists
+    import struct
+
+    import numpy as np
+
+    from nmslib_amd import indexpack
+    from tests import fixtures as F
+
+    path = F.index_path("tiny_l2")
+    ix = indexpack.read_index(path)
+    h, levels = ix["header"], ix["levels"]
+    data = bytearray(open(path, "rb").read())
+    # walk the per-node upper-level blocks after the level-0 block
+    pos = indexpack.HEADER.size + h["n"] * h["elem"]
+    target = None
+    for i in range(h["n"]):
+        (sm,) = struct.unpack_from("<I", data, pos)
+        pos += 4
+        if sm and target is None:
+            cnt = struct.unpack_from("<i", data, pos)[0]  # level-1 list of node i
+            if cnt > 0:
+                target = pos + 4  # its first neighbour
+        pos += sm
+    assert target is not None
+    level0_only = int(np.nonzero(levels == 0)[0][0])
+    struct.pack_into("<i", data, target, level0_only)
+    bad = tmp_path / "bad_level.hnsw"
+    bad.write_bytes(bytes(data))
+    idx = nmslib.init(space="l2")
+    with pytest.raises(RuntimeError, match="has no list at level 1"):
+        idx.loadIndex(str(bad))
+    idx.loadIndex(path)  # the untouched file loads
