@@ -65,3 +65,15 @@ def test_getitem_and_get_distance():
     linf = _index("linf")
     linf.addDataPointBatch(x)
     assert linf.getDistance(0, 3) == 8.0
+
+
+def test_import_nmslib_alias_is_the_device_module():
+    # `import nmslib` code is a drop-in: the reference module's names resolve
+    # to this library's (python_bindings/nmslib.cc:458-789)
+    import nmslib as alias
+    assert alias.init is nmslib.init
+    assert alias.DataType.DENSE_VECTOR == nmslib.DataType.DENSE_VECTOR
+    idx = alias.init(method="hnsw", space="l2")
+    assert isinstance(idx, nmslib.FloatIndex)
+    assert alias.addDataPoint(idx, 7, [0.1, 0.2]) == 0
+    assert alias.getDataPointQty(idx) == 1
