@@ -118,6 +118,23 @@ int nmsamd_knn_query_batch_device(nmsamd_index* h, const float* d_q, int64_t nq,
                                   int32_t* d_ids, float* d_dists, int32_t* d_counts,
                                   void* stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same search as nmsamd_knn_query_batch_device, same
+ * arguments and results, but it calls no hipMalloc / hipFree: its scratch
+ * (work counter, error word, epochs, visited marks, SearchOld heaps) is a
+ * file-scope device array of 256 MiB per GPU, held until the process ends.
+ * As many queries run at once as visited slots of n + 1 bytes (plus the
+ * SearchOld heap when that search runs) fit in it; an index too large for
+ * one slot is refused.  The index must already be in device memory, and a
+ * BF16 / FP8 traversal's converted vectors must exist (a search through
+ * another entry point makes both).  No counters are collected.  Every call
+ * on one GPU shares its array and clears what it uses first: calls that
+ * overlap in time on one GPU race. */
+int nmsamd_knn_query_batch_device_noalloc(nmsamd_index* h, const float* d_q, int64_t nq,
+                                          int32_t k, int32_t* d_ids, float* d_dists,
+                                          int32_t* d_counts, void* stream);
+
 /* Uneven batch: per-query k[i] and ef[i] (host arrays) in one persistent
  * launch (no reference counterpart: the reference runs one ef per index and
  * one k per call).  Output rows have stride max_k = max(k[i]). */

@@ -62,6 +62,7 @@ SIGNATURES = {
     "nmsamd_set_query_time_params": (C.c_int, [_P, C.POINTER(C.c_char_p), C.c_int]),
     "nmsamd_knn_query_batch": (C.c_int, [_P, _P, _I64, _I64, _I32, _I32, _P, _P, _P]),
     "nmsamd_knn_query_batch_device": (C.c_int, [_P, _P, _I64, _I32, _P, _P, _P, _P]),
+    "nmsamd_knn_query_batch_device_noalloc": (C.c_int, [_P, _P, _I64, _I32, _P, _P, _P, _P]),
     "nmsamd_knn_query_batch_uneven": (C.c_int, [_P, _P, _I64, _I64, _P, _P, _I32, _P, _P, _P]),
     "nmsamd_size": (_I64, [_P]),
     "nmsamd_index_info": (C.c_int, [_P, _P]),
@@ -335,6 +336,14 @@ class FloatIndex:
                             counts_ptr: int, stream: int = 0):
         """Queries and outputs already in device memory (raw pointers)."""
         _check(load_library().nmsamd_knn_query_batch_device(
+            self._h, C.c_void_p(q_ptr), int(nq), int(k), C.c_void_p(ids_ptr),
+            C.c_void_p(dists_ptr), C.c_void_p(counts_ptr), C.c_void_p(stream or None)))
+
+    def knnQueryBatchDeviceNoAlloc(self, q_ptr: int, nq: int, k: int, ids_ptr: int,
+                                   dists_ptr: int, counts_ptr: int, stream: int = 0):
+        """knnQueryBatchDevice without device allocations: the library's
+        per-GPU scratch array holds the search state (see the C header)."""
+        _check(load_library().nmsamd_knn_query_batch_device_noalloc(
             self._h, C.c_void_p(q_ptr), int(nq), int(k), C.c_void_p(ids_ptr),
             C.c_void_p(dists_ptr), C.c_void_p(counts_ptr), C.c_void_p(stream or None)))
 

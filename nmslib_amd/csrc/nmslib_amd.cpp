@@ -155,6 +155,8 @@ struct nmsamd_index {
   DevBuf<int32_t> d_bpi;
   bool pending = false;
   hipStream_t last_stream = nullptr;  // stream of the last search launch (ev1 follows it)
+  int* last_err = nullptr;            // error word of the last search launch
+  bool last_collect = false;          // the last search launch collected counters
   int64_t last_nq = 0;
   SearchStats last_stats{};
   int64_t last_launch[5] = {0, 0, 0, 0, 0};  // kernel mode, blocks, waves/block, LDS/wave, entries
@@ -580,11 +582,52 @@ void follow_last_launch(nmsamd_index* h, hipStream_t s) {
   h->last_stream = s;
 }
 
+// Device memory a launch may use instead of the handle's own buffers: the
+// work counter and error word, the slots' epochs and visited marks and the
+// SearchOld heaps are laid out in [base, base + bytes).
+struct ExtScratch {
+  unsigned char* base;
+  size_t bytes;
+};
+
+// Scratch of nmsamd_knn_query_batch_device_noalloc (256 MiB per GPU).
+constexpr size_t kNoAllocScratchBytes = (size_t)256 << 20;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+// The address of GPU `device`'s copy of `scratch`, taken once per device with
+// that device current.
+unsigned char* scratch_on(int device) {
+  if (device < 0 || device >= kMaxDevices) fail(NMSAMD_EDEVICE, "device ordinal out of range");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = 0;
+    hip_check(hipGetDevice(&cur), "hipGetDevice");
+    if (cur != device) hip_check(hipSetDevice(device), "hipSetDevice");
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    hip_check(e, "hipGetSymbolAddress(scratch)");
+    if (!p) fail(NMSAMD_EDEVICE, "hipGetSymbolAddress(scratch) returned no address");
+    g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+  }
+  return g_scratch_ptr[device];
+}
+
 // Launches the search of nq queries whose inputs/outputs are device pointers;
 // k_of / ef_of are optional device arrays whose maxima are max_k / max_ef.
+// With `ext`, the launch allocates nothing: its scratch comes from ext (the
+// resident slots are as many as fit) and no counters are collected.
 void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, const int32_t* k_of,
                 const int32_t* ef_of, int max_k, int max_ef, int32_t* ids, float* dists,
-                int32_t* counts, hipStream_t s) {
+                int32_t* counts, hipStream_t s, const ExtScratch* ext = nullptr) {
   if (!h->loaded) fail(NMSAMD_EINVAL, "Must call createIndex or loadIndex before this method");
   // Hnsw::Search (hnsw.cc:724): SearchOld for algoType=old or hybrid with ef
   // >= 1000; beam mode replaces V1Merge and the hybrid switch for the batch
@@ -605,7 +648,13 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   if ((size_t)lds > kLdsPerCu)
     fail(NMSAMD_EINVAL, "efSearch / k too large: the search array of " + std::to_string(cap) +
                             " items exceeds the LDS of one wave");
-  ensure_lowp(h);
+  if (ext) {
+    if ((h->storage == kBF16 && !h->d_vec16.p) || (h->storage == kFP8 && !h->d_vec8.p))
+      fail(NMSAMD_EINVAL, "the reduced-precision copy of the vectors is made by the first "
+                          "search in that precision: run one through another entry point first");
+  } else {
+    ensure_lowp(h);
+  }
   // queries per block and resident blocks per CU from the LDS footprint; the
   // grid is persistent: as many resident queries as the device holds, at most
   // one per query.  Beam batches take the beam-only kernels: where the LDS
@@ -629,29 +678,61 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   const int mode = !beam ? kModeAll
                          : (pair ? kModeBeamPair : (wpb * per_cu <= 4 ? kModeBeamWide : kModeBeam));
   int64_t max_blocks = (int64_t)h->cu_count * per_cu;
-  // visited marks take n bytes per slot: keep them under ~16 GiB
-  const int64_t vis_budget = (int64_t)16 << 30;
-  const int64_t slot_cap = std::max<int64_t>(wpb, vis_budget / std::max<int64_t>(h->n, 1));
+  const int64_t heap_cap = any_old ? std::max<int64_t>(65536, 64 * (int64_t)max_ef) : 0;
+  const int64_t vstride = ((int64_t)h->n + 1 + 255) / 256 * 256;
+  // visited marks take n bytes per slot: keep them under ~16 GiB, or within
+  // the external scratch (256 B of counters, the epochs, then per slot its
+  // marks and SearchOld heap)
+  int64_t slot_cap = std::max<int64_t>(wpb, ((int64_t)16 << 30) / std::max<int64_t>(h->n, 1));
+  if (ext) {
+    const int64_t per_slot = vstride + heap_cap * 8 + 1;
+    slot_cap = ((int64_t)ext->bytes - 512) / per_slot;
+    if (slot_cap < wpb)
+      fail(NMSAMD_EINVAL, "the no-allocation scratch of " + std::to_string(ext->bytes >> 20) +
+                              " MiB holds no search slot for an index of " + std::to_string(h->n) +
+                              " points" + (any_old ? " with the SearchOld heap" : ""));
+  }
   max_blocks = std::max<int64_t>(1, std::min<int64_t>(max_blocks, slot_cap / wpb));
   const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((nq + wpb - 1) / wpb, max_blocks));
   const int slots = blocks * wpb;
-  ensure_slots(h, slots);
-  int64_t heap_cap = 0;
-  if (any_old) {
-    heap_cap = std::max<int64_t>(65536, 64 * (int64_t)max_ef);
-    h->d_oldk.ensure((size_t)slots * heap_cap);
-    h->d_oldi.ensure((size_t)slots * heap_cap);
-  }
+  int* work = nullptr;
+  DevIndex ix = dev_index(h);
+  float* old_key = nullptr;
+  int32_t* old_id = nullptr;
   follow_last_launch(h, s);
-  h->d_work.ensure(2);
-  hip_check(launch_fill_i32(h->d_work.p, 0, 2, s), "fill");
-  if (h->collect) {
+  if (ext) {
+    const int64_t ep_bytes = ((int64_t)slots + 255) / 256 * 256;
+    work = reinterpret_cast<int*>(ext->base);
+    ix.epoch = ext->base + 256;
+    ix.visited = ext->base + 256 + ep_bytes;
+    ix.vstride = vstride;
+    old_key = reinterpret_cast<float*>(ix.visited + (int64_t)slots * vstride);
+    old_id = reinterpret_cast<int32_t*>(old_key + (int64_t)slots * heap_cap);
+    // the scratch keeps the last call's contents (possibly another index's
+    // layout): counters, epochs and marks start from zero
+    const int64_t clear = 256 + ep_bytes + (int64_t)slots * vstride;
+    hip_check(launch_fill_i32(work, 0, (int)(clear / 4), s), "fill");
+  } else {
+    ensure_slots(h, slots);
+    ix = dev_index(h);
+    if (any_old) {
+      h->d_oldk.ensure((size_t)slots * heap_cap);
+      h->d_oldi.ensure((size_t)slots * heap_cap);
+    }
+    old_key = h->d_oldk.p;
+    old_id = h->d_oldi.p;
+    h->d_work.ensure(2);
+    work = h->d_work.p;
+    hip_check(launch_fill_i32(work, 0, 2, s), "fill");
+  }
+  const bool collect = h->collect && !ext;
+  if (collect) {
     h->d_stats.ensure(1);
     hip_check(launch_fill_i32(reinterpret_cast<int*>(h->d_stats.p), 0,
                               sizeof(SearchStats) / sizeof(int), s), "fill");
   }
   SearchArgs a{};
-  a.ix = dev_index(h);
+  a.ix = ix;
   a.queries = d_q;
   a.nq = nq;
   a.k_of = k_of;
@@ -665,19 +746,19 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   a.out_dists = dists;
   a.out_counts = counts;
   a.out_internal = nullptr;
-  a.work = h->d_work.p;
-  a.err = h->d_work.p + 1;
-  a.stats = h->collect ? h->d_stats.p : nullptr;
-  if (h->collect) h->d_qticks.ensure((size_t)nq);
-  a.q_ticks = h->collect ? h->d_qticks.p : nullptr;
+  a.work = work;
+  a.err = work + 1;
+  a.stats = collect ? h->d_stats.p : nullptr;
+  if (collect) h->d_qticks.ensure((size_t)nq);
+  a.q_ticks = collect ? h->d_qticks.p : nullptr;
   h->last_nq = nq;
   a.storage = h->storage;
   a.rerank = rerank ? h->rerank_mult : 0;
   a.beam = h->beam;
   a.ent = ent;
   a.algo = h->algo == kOld ? kAlgoOld : (h->algo == kV1Merge ? kAlgoV1Merge : kAlgoHybrid);
-  a.old_heap_key = h->d_oldk.p;
-  a.old_heap_id = h->d_oldi.p;
+  a.old_heap_key = old_key;
+  a.old_heap_id = old_id;
   a.old_heap_cap = heap_cap;
   hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
   hip_check(launch_search(a, blocks, wpb, lds_launch, mode, s), "search kernel launch");
@@ -687,6 +768,8 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   h->last_launch[2] = wpb;
   h->last_launch[3] = lds_launch;
   h->last_launch[4] = ent;
+  h->last_err = work + 1;
+  h->last_collect = collect;
   h->pending = true;
 }
 
@@ -694,7 +777,7 @@ void finish_stats(nmsamd_index* h) {
   if (!h->pending) return;
   h->pending = false;
   int err = 0;
-  hip_check(hipMemcpy(&err, h->d_work.p + 1, sizeof(int), hipMemcpyDeviceToHost), "status");
+  hip_check(hipMemcpy(&err, h->last_err, sizeof(int), hipMemcpyDeviceToHost), "status");
   if (err & 2)
     fail(NMSAMD_ERUNTIME, "internal error: the search state held a node id outside the index");
   if (err)
@@ -702,7 +785,7 @@ void finish_stats(nmsamd_index* h) {
   float ms = 0;
   hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
   h->last_ms = ms;
-  if (h->collect)
+  if (h->last_collect)
     hip_check(hipMemcpy(&h->last_stats, h->d_stats.p, sizeof(SearchStats), hipMemcpyDeviceToHost),
               "stats");
 }
@@ -1089,6 +1172,30 @@ int nmsamd_knn_query_batch_device(nmsamd_index* h, const float* d_q, int64_t nq,
     if (nq == 0) return;
     hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : h->stream;
     run_search(h, d_q, nq, k, (int)h->ef, nullptr, nullptr, k, (int)h->ef, d_ids, d_dists, d_counts, s);
+  });
+}
+
+// The no-allocation variant of nmsamd_knn_query_batch_device: its scratch
+// (work counter, error word, epochs, visited marks, SearchOld heaps) is this
+// file-scope device array, one copy per GPU, held until the process ends.
+// Host code addresses a GPU's copy only through the pointer
+// hipGetSymbolAddress returns for it on that GPU (scratch_on()).
+int nmsamd_knn_query_batch_device_noalloc(nmsamd_index* h, const float* d_q, int64_t nq,
+                                          int32_t k, int32_t* d_ids, float* d_dists,
+                                          int32_t* d_counts, void* stream) {
+  return guarded([&] {
+    if (!h) fail(NMSAMD_EINVAL, "null handle");
+    std::lock_guard<std::mutex> g(h->mu);
+    if (!h->loaded) fail(NMSAMD_EINVAL, "Must call createIndex or loadIndex before this method");
+    if (!h->on_device)
+      fail(NMSAMD_EINVAL, "the index is not in device memory yet: the no-allocation entry point "
+                          "does not upload it (any other search does)");
+    if (nq < 0 || k <= 0) fail(NMSAMD_EINVAL, "nq must be non-negative and k positive");
+    if (nq == 0) return;
+    hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : h->stream;
+    const ExtScratch ext{scratch_on(h->device), kNoAllocScratchBytes};
+    run_search(h, d_q, nq, k, (int)h->ef, nullptr, nullptr, k, (int)h->ef, d_ids, d_dists,
+               d_counts, s, &ext);
   });
 }
 
