@@ -77,6 +77,13 @@ int nmsamd_get_data_point(nmsamd_index* h, int64_t pos, float* out, int64_t cap,
  * reference's SSE summation order. */
 int nmsamd_get_distance(nmsamd_index* h, int64_t pos1, int64_t pos2, float* out);
 
+/* The space's IndexTimeDistance between two vectors of dimension dim (what
+ * the reference's query server's getDistance returns,
+ * query_server/cpp_client_server/QueryService_server.cpp:276-312): the same
+ * functions as nmsamd_get_distance, over caller vectors. */
+int nmsamd_vector_distance(nmsamd_index* h, const float* a, const float* b, int64_t dim,
+                           float* out);
+
 /* IndexWrapper::createIndex(index_params, print_progress) (nmslib.cc:89-96 ->
  * Hnsw::CreateIndex, hnsw.cc:183-470): builds the HNSW graph over the data
  * added so far on the host CPU and keeps it as the optimized index, ready for
@@ -199,14 +206,33 @@ int nmsamd_last_phase_us(const nmsamd_index* h, double* out8);
  * neighbour entries per step.  Diagnostic; no reference counterpart. */
 int nmsamd_last_launch(const nmsamd_index* h, int64_t* out5);
 
-/* Exact k-NN over the loaded index's FP32 vectors on MFMA: the device
- * counterpart of the reference's seq_search method (method/seqsearch.cc:
- * 143-177) and of its gold standard (gold_standard.h:151-175: ranked by
- * (distance, id)).  Distances as the HNSW search reports them (squared L2,
- * 1 - cos, -dot), computed as |x|^2 + |q|^2 - 2 q.x for l2; dims <= 128,
+/* Exact k-NN over the loaded index's FP32 vectors: the device counterpart of
+ * the reference's seq_search method (method/seqsearch.cc:143-177) and of its
+ * gold standard (gold_standard.h:151-175: ranked by (distance, id)).  A
+ * matrix-core screen keeps the best kc = min(256, max(k + 32, 2k)) candidates
+ * of each query; their distances are recomputed in FP32 in the reference's
+ * summation order and ranked by (distance, internal id).  A query whose
+ * screen could have missed an answer (its kc-th screen distance does not
+ * clear the k-th exact one by twice the screen's error bound) is searched
+ * again by an exact scan of every point, so the result is exact.  Distances
+ * are those the HNSW search reports (squared L2, 1 - cos, -dot, AVX order);
+ * optimized l2 / cosinesimil / negdotprod indices, dims <= 128,
  * 1 <= k <= 128.  Output [nq][k] rows, counts[nq]. */
 int nmsamd_brute_force(nmsamd_index* h, const float* q, int64_t nq, int64_t dim, int32_t k,
                        int32_t* ids, float* dists, int32_t* counts);
+
+/* SeqSearch::Search (method/seqsearch.cc:143-177) over the index's stored
+ * vectors: the same exact search ranked and reported in the space's own
+ * distance as the reference's seq_search computes it -- l2: L2NormSIMD (the
+ * square root of the SSE-order sum, distcomp_lp.cc:304-371); cosinesimil:
+ * CosineSimilarity (distcomp_scalar.cc:267-271) of the stored (normalised)
+ * vectors; negdotprod: -ScalarProductSIMD -- ties by internal id. */
+int nmsamd_seq_search(nmsamd_index* h, const float* q, int64_t nq, int64_t dim, int32_t k,
+                      int32_t* ids, float* dists, int32_t* counts);
+
+/* No counterpart in the reference (diagnostic): how many queries of the last
+ * brute force / seq_search needed the exact scan of every point. */
+int nmsamd_last_bruteforce_rescans(const nmsamd_index* h, int64_t* out);
 
 /* ---- continuous-batching server ------------------------------------------
  * No reference counterpart in the library; it stands where the reference's

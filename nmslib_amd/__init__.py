@@ -63,6 +63,9 @@ SIGNATURES = {
     "nmsamd_knn_query_batch": (C.c_int, [_P, _P, _I64, _I64, _I32, _I32, _P, _P, _P]),
     "nmsamd_knn_query_batch_device": (C.c_int, [_P, _P, _I64, _I32, _P, _P, _P, _P]),
     "nmsamd_knn_query_batch_device_noalloc": (C.c_int, [_P, _P, _I64, _I32, _P, _P, _P, _P]),
+    "nmsamd_seq_search": (C.c_int, [_P, _P, _I64, _I64, _I32, _P, _P, _P]),
+    "nmsamd_vector_distance": (C.c_int, [_P, _P, _P, _I64, C.POINTER(C.c_float)]),
+    "nmsamd_last_bruteforce_rescans": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "nmsamd_knn_query_batch_uneven": (C.c_int, [_P, _P, _I64, _I64, _P, _P, _I32, _P, _P, _P]),
     "nmsamd_size": (_I64, [_P]),
     "nmsamd_index_info": (C.c_int, [_P, _P]),
@@ -149,6 +152,7 @@ class FloatIndex:
         _check(load_library().nmsamd_init(space.encode(), method.encode(), C.byref(h)))
         self._h = h
         self._dim = 0
+        self._algo = "hybrid"  # algoType as last set (hnsw.cc:491-501)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -160,6 +164,7 @@ class FloatIndex:
     def loadIndex(self, filename: str, load_data: bool = False):
         _check(load_library().nmsamd_load_index(self._h, os.fsencode(filename), int(bool(load_data))))
         self._dim = self.info()["dim"]
+        self._algo = "hybrid"
 
     def saveIndex(self, filename: str, save_data: bool = False):
         _check(load_library().nmsamd_save_index(self._h, os.fsencode(filename), int(bool(save_data))))
@@ -171,6 +176,7 @@ class FloatIndex:
         arr = (C.c_char_p * max(1, len(kv)))(*[s.encode() for s in kv])
         _check(load_library().nmsamd_create_index(self._h, arr, len(kv), int(bool(print_progress))))
         self._dim = self.info()["dim"]
+        self._algo = "hybrid"
 
     def addDataPoint(self, id, data):
         """Appends one point with object id `id`; returns its position
@@ -210,6 +216,22 @@ class FloatIndex:
         kv = _params_to_kv(params)
         arr = (C.c_char_p * max(1, len(kv)))(*[s.encode() for s in kv])
         _check(load_library().nmsamd_set_query_time_params(self._h, arr, len(kv)))
+        self._algo = "hybrid"
+        for p in kv:
+            if p.split("=", 1)[0] == "algoType":
+                self._algo = p.split("=", 1)[1].lower()
+
+    def vectorDistance(self, a, b) -> float:
+        """The space's own distance between two vectors (IndexTimeDistance,
+        as the reference's query server's getDistance computes it)."""
+        a = np.ascontiguousarray(a, np.float32).ravel()
+        b = np.ascontiguousarray(b, np.float32).ravel()
+        if a.shape != b.shape:
+            raise ValueError("vectors differ in dimension")
+        out = C.c_float()
+        _check(load_library().nmsamd_vector_distance(self._h, _ptr(a), _ptr(b), a.shape[0],
+                                                     C.byref(out)))
+        return float(out.value)
 
     def setPrecision(self, storage: str = "fp32", rerank_mult: int = 0):
         """Vector storage used while traversing ('fp32', 'bf16', 'fp8') and the
@@ -304,8 +326,9 @@ class FloatIndex:
         return ids, dists
 
     def bruteForce(self, queries, k: int = 10):
-        """Exact k-NN on the device (MFMA), the counterpart of the reference's
-        seq_search: (ids[nq, k], dists[nq, k], counts[nq]) by (distance, id)."""
+        """Exact k-NN on the device (matrix-core screen, exact FP32 re-scoring):
+        (ids[nq, k], dists[nq, k], counts[nq]) by (distance, id) in the
+        distance the HNSW search reports."""
         q = _as_queries(queries, self._dim)
         nq = q.shape[0]
         ids = np.empty((nq, k), np.int32)
@@ -314,6 +337,25 @@ class FloatIndex:
         _check(load_library().nmsamd_brute_force(self._h, _ptr(q), nq, q.shape[1], int(k),
                                                  _ptr(ids), _ptr(dists), _ptr(counts)))
         return ids, dists, counts
+
+    def seqSearch(self, queries, k: int = 10):
+        """The reference's seq_search on the device: exact k-NN in the space's
+        own distance (l2 with the square root, SSE summation order), ranked by
+        (distance, id): (ids[nq, k], dists[nq, k], counts[nq])."""
+        q = _as_queries(queries, self._dim)
+        nq = q.shape[0]
+        ids = np.empty((nq, k), np.int32)
+        dists = np.empty((nq, k), np.float32)
+        counts = np.empty(nq, np.int32)
+        _check(load_library().nmsamd_seq_search(self._h, _ptr(q), nq, q.shape[1], int(k),
+                                                _ptr(ids), _ptr(dists), _ptr(counts)))
+        return ids, dists, counts
+
+    def lastBruteForceRescans(self) -> int:
+        """Queries of the last bruteForce / seqSearch that took the exact scan."""
+        out = C.c_int64()
+        _check(load_library().nmsamd_last_bruteforce_rescans(self._h, C.byref(out)))
+        return int(out.value)
 
     def knnQueryBatchUneven(self, queries, ks, efs):
         """One persistent launch with per-query k and ef."""

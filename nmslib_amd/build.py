@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libnmslib_amd.so")
 SOURCES = ["search_f32.hip", "search_bf16.hip", "search_fp8.hip", "launch.hip",
-           "lowp_convert.hip", "bruteforce.hip", "nmslib_amd.cpp", "hnsw_build.cpp"]
+           "lowp_convert.hip", "bruteforce.hip", "bruteforce_exact.hip", "nmslib_amd.cpp", "hnsw_build.cpp"]
 # host-only sources: the index builder (x86 SSE, no device code)
 HOST_SOURCES = {"hnsw_build.cpp"}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -24,13 +24,30 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=o
          "-Wall", "-Wno-unused-function"]
 
 
-def build(verbose: bool = False) -> str:
+def _stale(src: str, obj: str) -> bool:
+    """The object is older than its source or any header of csrc/ and include/."""
+    if not os.path.exists(obj):
+        return True
+    deps = [os.path.join(CSRC, src)]
+    deps += [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    deps += [os.path.join(inc, f) for f in os.listdir(inc)]
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build(verbose: bool = False, incremental: bool = False) -> str:
+    """Compiles every source (incremental=True: only those whose object is
+    out of date) and links the library."""
     objs = []
     builddir = os.path.join(HERE, "_build")
     os.makedirs(builddir, exist_ok=True)
     procs = []
     for src in SOURCES:
         obj = os.path.join(builddir, src + ".o")
+        if incremental and not _stale(src, obj):
+            objs.append(obj)
+            continue
         cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
         if src in HOST_SOURCES:
             cmd = [CXX, *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
@@ -59,4 +76,4 @@ def build(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, incremental="--incremental" in sys.argv))

@@ -1,7 +1,9 @@
-// Exact k-NN by brute force on MFMA: the device counterpart of the
-// reference's seq_search method (similarity_search/src/method/seqsearch.cc:
-// 143-177), used as ground truth for workloads too large for the host and as
-// the dense-contraction kernel of the path.
+// Exact k-NN by brute force, first stage (the screen) on MFMA: the device
+// counterpart of the reference's seq_search method
+// (similarity_search/src/method/seqsearch.cc:143-177) is this screen followed
+// by the exact re-scoring of bruteforce_exact.hip.  It is the ground truth for
+// workloads too large for the host and the dense-contraction kernel of the
+// path.
 //
 // Kernel 1 (bf_tile_kernel): a wave owns 32 queries and one chunk of the base
 // vectors.  Per tile of 32 base vectors it computes the 32x32 block of dot
@@ -12,7 +14,7 @@
 // negdotprod: -q.x) and keeps a per-lane top-k in LDS.  Each query's two lanes
 // write their lists for the chunk.
 // Kernel 2 (bf_merge_kernel): one wave per query merges the sorted partial
-// lists by (distance, index) into the final k.
+// lists by (distance, index) into the query's kc candidates.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -141,16 +143,20 @@ __global__ void __launch_bounds__(64) bf_tile_kernel(BruteArgs a) {
   }
 }
 
-// one wave per query: k-way merge of nlists sorted lists of length k
+// one wave per list group: k-way merge of nlists sorted lists of length k by
+// (distance, internal id) into output row row_of[g] (g when row_of is null);
+// ids are object ids through ext_id, or internal ids when ext_id is null
 __global__ void __launch_bounds__(64) bf_merge_kernel(const float* part_d, const int32_t* part_i,
-                                                      int nlists, int k, int64_t nq,
+                                                      int nlists, int k, int64_t ngroups,
+                                                      const int32_t* row_of,
                                                       const int32_t* ext_id, int32_t* ids,
                                                       float* dists, int32_t* counts) {
-  const int64_t q = blockIdx.x;
+  const int64_t g = blockIdx.x;
   const int lane = threadIdx.x;
-  if (q >= nq) return;
-  const float* pd = part_d + q * nlists * k;
-  const int32_t* pi = part_i + q * nlists * k;
+  if (g >= ngroups) return;
+  const int64_t q = row_of ? row_of[g] : g;
+  const float* pd = part_d + g * nlists * k;
+  const int32_t* pi = part_i + g * nlists * k;
   int head = 0;  // this lane's position in list `lane`
   int found = 0;
   for (int r = 0; r < k; ++r) {
@@ -176,7 +182,7 @@ __global__ void __launch_bounds__(64) bf_merge_kernel(const float* part_d, const
     if (bi == 0x7fffffff) break;
     if (lane == bl) ++head;
     if (lane == 0) {
-      ids[q * k + r] = ext_id[bi];
+      ids[q * k + r] = ext_id ? ext_id[bi] : bi;
       dists[q * k + r] = bd;
     }
     ++found;
@@ -185,7 +191,7 @@ __global__ void __launch_bounds__(64) bf_merge_kernel(const float* part_d, const
     ids[q * k + r] = -1;
     dists[q * k + r] = 0.f;
   }
-  if (lane == 0) counts[q] = found;
+  if (lane == 0 && counts) counts[q] = found;
 }
 
 __global__ void bf_norm_kernel(const float* vec, float* out, int64_t n, int dstride) {
@@ -204,15 +210,17 @@ hipError_t launch_brute_norms(const float* vec, float* out, int64_t n, int dstri
   return hipGetLastError();
 }
 
-hipError_t launch_brute(const float* vec, const float* xnorm, const float* queries, int64_t n,
-                        int64_t nq, int dstride, int func, int k, int nchunks, float* part_d,
-                        int32_t* part_i, const int32_t* ext_id, int32_t* ids, float* dists,
-                        int32_t* counts, hipStream_t stream) {
-  if (dstride > 128) return hipErrorInvalidValue;
-  BruteArgs a{vec, xnorm, queries, n, nq, dstride, func, k, nchunks,
+// The screen: kc candidates per query by the matrix-core distance, merged
+// into cand_d / cand_i ([nq][kc], internal ids, ascending by (distance, id)).
+hipError_t launch_brute_screen(const float* vec, const float* xnorm, const float* queries,
+                               int64_t n, int64_t nq, int dstride, int func, int kc, int nchunks,
+                               float* part_d, int32_t* part_i, float* cand_d, int32_t* cand_i,
+                               hipStream_t stream) {
+  if (dstride > 128 || kc < 1 || kc > 256 || nchunks * 2 > 64) return hipErrorInvalidValue;
+  BruteArgs a{vec, xnorm, queries, n, nq, dstride, func, kc, nchunks,
               (n + nchunks - 1) / nchunks, part_d, part_i};
   a.chunk_len = (a.chunk_len + 31) / 32 * 32;
-  const size_t lds = (size_t)64 * k * 8 + 32 * 4;
+  const size_t lds = (size_t)64 * kc * 8 + 32 * 4;
   if (lds > 64 * 1024) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(bf_tile_kernel),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -223,7 +231,16 @@ hipError_t launch_brute(const float* vec, const float* xnorm, const float* queri
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(bf_merge_kernel, dim3((unsigned)nq), dim3(64), 0, stream, part_d, part_i,
-                     nchunks * 2, k, nq, ext_id, ids, dists, counts);
+                     nchunks * 2, kc, nq, nullptr, nullptr, cand_i, cand_d, nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_brute_merge(const float* part_d, const int32_t* part_i, int nlists, int k,
+                              int64_t rows, const int32_t* row_of, const int32_t* ext_id,
+                              int32_t* ids, float* dists, int32_t* counts, hipStream_t stream) {
+  if (nlists < 1 || nlists > 64 || rows <= 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(bf_merge_kernel, dim3((unsigned)rows), dim3(64), 0, stream, part_d, part_i,
+                     nlists, k, rows, row_of, ext_id, ids, dists, counts);
   return hipGetLastError();
 }
 

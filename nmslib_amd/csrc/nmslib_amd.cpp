@@ -151,8 +151,13 @@ struct nmsamd_index {
   DevBuf<int32_t> d_oldi;
   DevBuf<int32_t> d_order;  // uneven batches: query order
   const int32_t* order_dev = nullptr;
-  DevBuf<float> d_xnorm, d_bq, d_bpd;  // brute force: |x|^2, padded queries, partial lists
-  DevBuf<int32_t> d_bpi;
+  // brute force: |x|^2, padded queries, per-query screen error bounds, partial
+  // lists, screened candidates, listed queries (+ their count)
+  DevBuf<float> d_xnorm, d_bq, d_beps, d_bpd, d_bcd;
+  DevBuf<int32_t> d_bpi, d_bci, d_bflag;
+  bool vec_bounds_known = false;
+  double vec_norm_max = 0, vec_int_absmax = -1;
+  int64_t last_bf_rescanned = 0;
   bool pending = false;
   hipStream_t last_stream = nullptr;  // stream of the last search launch (ev1 follows it)
   int* last_err = nullptr;            // error word of the last search launch
@@ -479,6 +484,7 @@ void upload_index(nmsamd_index* h) {
   h->d_vec8.release();
   h->d_scale8.release();
   h->d_xnorm.release();
+  h->vec_bounds_known = false;
   h->d_visited.release();
   h->d_epoch.release();
   h->slots = 0;
@@ -1024,6 +1030,16 @@ int nmsamd_get_distance(nmsamd_index* h, int64_t pos1, int64_t pos2, float* out)
     const int space = h->expected_func == 1 ? (int)kBuildL2 : h->expected_func;
     *out = space_distance(space, reinterpret_cast<const float*>(a.data() + 16),
                           reinterpret_cast<const float*>(b.data() + 16), (a.size() - 16) / 4);
+  });
+}
+
+int nmsamd_vector_distance(nmsamd_index* h, const float* a, const float* b, int64_t dim,
+                           float* out) {
+  return guarded([&] {
+    if (!h || !a || !b || !out) fail(NMSAMD_EINVAL, "null argument");
+    if (dim <= 0) fail(NMSAMD_EINVAL, "vectors need a positive dimension");
+    const int space = h->expected_func == 1 ? (int)kBuildL2 : h->expected_func;
+    *out = space_distance(space, a, b, (size_t)dim);
   });
 }
 
@@ -1602,72 +1618,191 @@ int nmsamd_server_close(nmsamd_server* s) {
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
-// Exact k-NN by brute force on MFMA (the device seq_search,
-// similarity_search/src/method/seqsearch.cc:143-177).
+// Exact k-NN by brute force (the device seq_search,
+// similarity_search/src/method/seqsearch.cc:143-177): a matrix-core screen
+// keeps kc candidates per query, their FP32 distances are recomputed in the
+// reference's summation order and ranked by (distance, internal id), and a
+// query whose certificate fails (the screen's error could hide an answer) is
+// searched again by an exact scan of every point.
 namespace nmsamd {
 hipError_t launch_brute_norms(const float* vec, float* out, int64_t n, int dstride,
                               hipStream_t stream);
-hipError_t launch_brute(const float* vec, const float* xnorm, const float* queries, int64_t n,
-                        int64_t nq, int dstride, int func, int k, int nchunks, float* part_d,
-                        int32_t* part_i, const int32_t* ext_id, int32_t* ids, float* dists,
-                        int32_t* counts, hipStream_t stream);
+hipError_t launch_brute_screen(const float* vec, const float* xnorm, const float* queries,
+                               int64_t n, int64_t nq, int dstride, int func, int kc, int nchunks,
+                               float* part_d, int32_t* part_i, float* cand_d, int32_t* cand_i,
+                               hipStream_t stream);
+hipError_t launch_brute_merge(const float* part_d, const int32_t* part_i, int nlists, int k,
+                              int64_t rows, const int32_t* row_of, const int32_t* ext_id,
+                              int32_t* ids, float* dists, int32_t* counts, hipStream_t stream);
+hipError_t launch_brute_rescore(const DevIndex& ix, const float* queries, int64_t nq, int kc,
+                                const float* cand_d, const int32_t* cand_i, const float* eps,
+                                int k, int func, int sqrt_out, int32_t* ids, float* dists,
+                                int32_t* counts, int32_t* flagged, int* nflag,
+                                hipStream_t stream);
+hipError_t launch_brute_scan(const DevIndex& ix, const float* queries, const int32_t* rows,
+                             const int* nrows, int nrows_max, int k, int func, int nchunks,
+                             float* part_d, int32_t* part_i, hipStream_t stream);
 }  // namespace nmsamd
+
+namespace {
+
+// Facts about the stored vectors the screen's error bound needs: the largest
+// norm, and whether every coordinate is an integer (and its largest magnitude).
+void ensure_vector_bounds(nmsamd_index* h) {
+  if (h->vec_bounds_known) return;
+  double xmax2 = 0, amax = 0;
+  bool ints = true;
+  for (int64_t i = 0; i < (int64_t)h->n; ++i) {
+    const float* v = reinterpret_cast<const float*>(h->block.data() + i * h->elem + 16);
+    double s = 0;
+    for (int j = 0; j < h->dim; ++j) {
+      const double t = v[j];
+      s += t * t;
+      amax = std::max(amax, std::fabs(t));
+      ints = ints && std::nearbyint(t) == t;
+    }
+    xmax2 = std::max(xmax2, s);
+  }
+  h->vec_norm_max = std::sqrt(xmax2);
+  h->vec_int_absmax = ints ? amax : -1.0;
+  h->vec_bounds_known = true;
+}
+
+// mode 0: the index's own search distance (what knnQueryBatch reports);
+// mode 1: the space's distance as seq_search computes it (l2 with the square
+// root in the SSE order, CosineSimilarity, -ScalarProductSIMD)
+void run_bruteforce(nmsamd_index* h, const float* q, int64_t nq, int64_t dim, int32_t k,
+                    int32_t* ids, float* dists, int32_t* counts, int mode) {
+  require_index(h);
+  if (dim != h->dim) fail(NMSAMD_EINVAL, "query dimension does not match the index");
+  if (h->func == kL1 || h->func == kLInf || h->regular)
+    fail(NMSAMD_EINVAL, "brute force on matrix cores covers optimized l2, cosinesimil and "
+                        "negdotprod indices");
+  if (h->dstride > 128) fail(NMSAMD_EINVAL, "brute force supports dimensions up to 128");
+  if (k < 1 || k > 128) fail(NMSAMD_EINVAL, "brute force k must be in 1..128");
+  if (nq <= 0) return;
+  const bool l2 = h->func == kL2Sqr16 || h->func == kL2Sqr;
+  const bool cos = h->func == kNormCos;
+  const int exact_func = mode == 0 ? h->func : (l2 ? kSpaceL2 : (cos ? kSpaceCos : kSpaceNegDot));
+  hipStream_t s = h->stream;
+  ensure_vector_bounds(h);
+  if (!h->d_xnorm.p) {
+    h->d_xnorm.ensure((size_t)std::max<int64_t>(h->n, 1));
+    hip_check(launch_brute_norms(h->d_vec.p, h->d_xnorm.p, h->n, h->dstride, s), "norms");
+  }
+  // queries padded to the index row stride, normalised for cosine as the
+  // index vectors are (hnsw.h:486-497); the screen's error bound per query:
+  // an FP32 dot product of d terms is within (d + 2) u sum |q_i x_i| of the
+  // exact one, the norms likewise, so |screen - exact| <= 4 (d + 8) u S with
+  // S = (|x|max + |q|)^2 for l2 and |x|max |q| for the dot products; both the
+  // screen and the reference-order FP32 value are within that of the exact
+  // value.  Integer coordinates whose every partial sum stays below 2^24 make
+  // the screen exact (bound 0).
+  std::vector<float> qp((size_t)nq * h->dstride, 0.f);
+  std::vector<float> eps((size_t)nq, 0.f);
+  const double u = std::ldexp(1.0, -24);
+  for (int64_t i = 0; i < nq; ++i) {
+    float* row = &qp[i * h->dstride];
+    std::memcpy(row, q + i * dim, sizeof(float) * dim);
+    if (cos) {
+      float sum = 0.f;
+      for (int64_t j = 0; j < dim; ++j) sum = sum + row[j] * row[j];
+      if (sum != 0.0f) {
+        const float inv = 1.0f / std::sqrt(sum);
+        for (int64_t j = 0; j < dim; ++j) row[j] = row[j] * inv;
+      }
+    }
+    double qn2 = 0, qa = 0;
+    bool qint = true;
+    for (int64_t j = 0; j < dim; ++j) {
+      qn2 += (double)row[j] * row[j];
+      qa = std::max(qa, (double)std::fabs(row[j]));
+      qint = qint && std::nearbyint(row[j]) == row[j];
+    }
+    const double qn = std::sqrt(qn2), X = h->vec_int_absmax;
+    const double lim = std::ldexp(1.0, 24);
+    const bool exact = !cos && X >= 0 && qint &&
+                       (l2 ? dim * (X * X + qa * qa) < lim : dim * X * qa < lim);
+    const double S = l2 ? (h->vec_norm_max + qn) * (h->vec_norm_max + qn) : h->vec_norm_max * qn + 1.0;
+    eps[i] = exact ? 0.f : (float)(4.0 * (dim + 8) * u * S);
+  }
+  const int kc = (int)std::min<int64_t>(std::max<int64_t>(h->n, 1),
+                                        std::min(256, std::max(k + 32, 2 * k)));
+  const int64_t qblocks = (nq + 31) / 32;
+  int nchunks = (int)std::max<int64_t>(1, std::min<int64_t>(16, (2048 + qblocks - 1) / qblocks));
+  while (nchunks > 1 && (int64_t)h->n / nchunks < 64) --nchunks;
+  h->d_bq.upload(qp.data(), qp.size(), s);
+  h->d_beps.upload(eps.data(), eps.size(), s);
+  h->d_bpd.ensure((size_t)nq * nchunks * 2 * kc);
+  h->d_bpi.ensure((size_t)nq * nchunks * 2 * kc);
+  h->d_bcd.ensure((size_t)nq * kc);
+  h->d_bci.ensure((size_t)nq * kc);
+  h->d_ids.ensure((size_t)nq * k);
+  h->d_dists.ensure((size_t)nq * k);
+  h->d_counts.ensure((size_t)nq);
+  h->d_bflag.ensure((size_t)nq + 1);
+  follow_last_launch(h, s);
+  hip_check(launch_fill_i32(h->d_bflag.p + nq, 0, 1, s), "fill");
+  hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
+  hip_check(launch_brute_screen(h->d_vec.p, h->d_xnorm.p, h->d_bq.p, h->n, nq, h->dstride,
+                                h->func, kc, nchunks, h->d_bpd.p, h->d_bpi.p, h->d_bcd.p,
+                                h->d_bci.p, s),
+            "brute force screen");
+  DevIndex ix = dev_index(h);
+  hip_check(launch_brute_rescore(ix, h->d_bq.p, nq, kc, h->d_bcd.p, h->d_bci.p, h->d_beps.p, k,
+                                 exact_func, mode == 1 && l2, h->d_ids.p, h->d_dists.p,
+                                 h->d_counts.p, h->d_bflag.p, h->d_bflag.p + nq, s),
+            "brute force rescore");
+  int nflag = 0;
+  hip_check(hipMemcpyAsync(&nflag, h->d_bflag.p + nq, sizeof(int), hipMemcpyDeviceToHost, s), "D2H");
+  hip_check(hipStreamSynchronize(s), "brute force");
+  h->last_bf_rescanned = nflag;
+  if (nflag > 0) {
+    // exact scans of the listed queries, then their chunk lists merged into
+    // their rows
+    const int sch = (int)std::max<int64_t>(1, std::min<int64_t>(64, h->n / 4096));
+    h->d_bpd.ensure((size_t)nflag * sch * k);
+    h->d_bpi.ensure((size_t)nflag * sch * k);
+    hip_check(launch_brute_scan(ix, h->d_bq.p, h->d_bflag.p, h->d_bflag.p + nq, nflag, k,
+                                exact_func, sch, h->d_bpd.p, h->d_bpi.p, s),
+              "brute force exact scan");
+    hip_check(launch_brute_merge(h->d_bpd.p, h->d_bpi.p, sch, k, nflag, h->d_bflag.p, h->d_ext.p,
+                                 h->d_ids.p, h->d_dists.p, h->d_counts.p, s),
+              "brute force merge");
+  }
+  hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
+  hip_check(hipMemcpyAsync(ids, h->d_ids.p, sizeof(int32_t) * nq * k, hipMemcpyDeviceToHost, s), "D2H");
+  hip_check(hipMemcpyAsync(dists, h->d_dists.p, sizeof(float) * nq * k, hipMemcpyDeviceToHost, s), "D2H");
+  hip_check(hipMemcpyAsync(counts, h->d_counts.p, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, s), "D2H");
+  hip_check(hipStreamSynchronize(s), "brute force");
+  float ms = 0;
+  hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
+  h->last_ms = ms;
+}
+
+}  // namespace
 
 extern "C" int nmsamd_brute_force(nmsamd_index* h, const float* q, int64_t nq, int64_t dim,
                                   int32_t k, int32_t* ids, float* dists, int32_t* counts) {
   return guarded([&] {
     if (!h) fail(NMSAMD_EINVAL, "null handle");
     std::lock_guard<std::mutex> g(h->mu);
-    require_index(h);
-    if (dim != h->dim) fail(NMSAMD_EINVAL, "query dimension does not match the index");
-    if (h->func == kL1 || h->func == kLInf || h->regular)
-      fail(NMSAMD_EINVAL, "brute force on matrix cores covers optimized l2, cosinesimil and "
-                          "negdotprod indices");
-    if (h->dstride > 128) fail(NMSAMD_EINVAL, "brute force supports dimensions up to 128");
-    if (k < 1 || k > 128) fail(NMSAMD_EINVAL, "brute force k must be in 1..128");
-    if (nq <= 0) return;
-    hipStream_t s = h->stream;
-    if (!h->d_xnorm.p) {
-      h->d_xnorm.ensure((size_t)std::max<int64_t>(h->n, 1));
-      hip_check(launch_brute_norms(h->d_vec.p, h->d_xnorm.p, h->n, h->dstride, s), "norms");
-    }
-    // queries padded to the index row stride, normalised for cosine as the
-    // index vectors are (hnsw.h:486-497)
-    std::vector<float> qp((size_t)nq * h->dstride, 0.f);
-    for (int64_t i = 0; i < nq; ++i) {
-      float* row = &qp[i * h->dstride];
-      std::memcpy(row, q + i * dim, sizeof(float) * dim);
-      if (h->func == kNormCos) {
-        float sum = 0.f;
-        for (int64_t j = 0; j < dim; ++j) sum = sum + row[j] * row[j];
-        if (sum != 0.0f) {
-          const float inv = 1.0f / std::sqrt(sum);
-          for (int64_t j = 0; j < dim; ++j) row[j] = row[j] * inv;
-        }
-      }
-    }
-    const int64_t qblocks = (nq + 31) / 32;
-    int nchunks = (int)std::max<int64_t>(1, std::min<int64_t>(16, (2048 + qblocks - 1) / qblocks));
-    while (nchunks > 1 && (int64_t)h->n / nchunks < 64) --nchunks;
-    h->d_bq.upload(qp.data(), qp.size(), s);
-    h->d_bpd.ensure((size_t)nq * nchunks * 2 * k);
-    h->d_bpi.ensure((size_t)nq * nchunks * 2 * k);
-    h->d_ids.ensure((size_t)nq * k);
-    h->d_dists.ensure((size_t)nq * k);
-    h->d_counts.ensure((size_t)nq);
-    follow_last_launch(h, s);
-    hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
-    hip_check(launch_brute(h->d_vec.p, h->d_xnorm.p, h->d_bq.p, h->n, nq, h->dstride, h->func, k,
-                           nchunks, h->d_bpd.p, h->d_bpi.p, h->d_ext.p, h->d_ids.p, h->d_dists.p,
-                           h->d_counts.p, s),
-              "brute force launch");
-    hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
-    hip_check(hipMemcpyAsync(ids, h->d_ids.p, sizeof(int32_t) * nq * k, hipMemcpyDeviceToHost, s), "D2H");
-    hip_check(hipMemcpyAsync(dists, h->d_dists.p, sizeof(float) * nq * k, hipMemcpyDeviceToHost, s), "D2H");
-    hip_check(hipMemcpyAsync(counts, h->d_counts.p, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, s), "D2H");
-    hip_check(hipStreamSynchronize(s), "brute force");
-    float ms = 0;
-    hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
-    h->last_ms = ms;
+    run_bruteforce(h, q, nq, dim, k, ids, dists, counts, 0);
+  });
+}
+
+extern "C" int nmsamd_seq_search(nmsamd_index* h, const float* q, int64_t nq, int64_t dim,
+                                 int32_t k, int32_t* ids, float* dists, int32_t* counts) {
+  return guarded([&] {
+    if (!h) fail(NMSAMD_EINVAL, "null handle");
+    std::lock_guard<std::mutex> g(h->mu);
+    run_bruteforce(h, q, nq, dim, k, ids, dists, counts, 1);
+  });
+}
+
+extern "C" int nmsamd_last_bruteforce_rescans(const nmsamd_index* h, int64_t* out) {
+  return guarded([&] {
+    if (!h || !out) fail(NMSAMD_EINVAL, "null argument");
+    *out = h->last_bf_rescanned;
   });
 }
