@@ -178,6 +178,14 @@ int nmsamd_set_beam(nmsamd_index* h, int beam);
  * reference counterpart. */
 int nmsamd_set_beam_waves(nmsamd_index* h, int waves);
 
+/* Vector loads in flight in the one-wave beam kernels: 1 = the plain kernel
+ * (two waves per SIMD; one FP32 / two BF16 / four FP8 vectors per lane pair
+ * and round), 2 = the wide one (one wave per SIMD; two FP32 / five BF16 / eight
+ * FP8 vectors per lane pair, at most four waves per CU), 0 (default) = wide
+ * where the LDS footprint already holds a CU to four waves.  Results do not
+ * depend on it.  No reference counterpart (an autotuner knob). */
+int nmsamd_set_beam_gather(nmsamd_index* h, int gather);
+
 /* Per-call counters of the last search (expansions, level-0 distance
  * evaluations, upper-level distance evaluations, adjacency ints read,
  * expansions that took the tie path) and the kernel time of the last search

@@ -74,6 +74,7 @@ SIGNATURES = {
     "nmsamd_collect_stats": (C.c_int, [_P, C.c_int]),
     "nmsamd_last_query_us": (C.c_int, [_P, _P, _I64]),
     "nmsamd_set_beam": (C.c_int, [_P, C.c_int]),
+    "nmsamd_set_beam_gather": (C.c_int, [_P, C.c_int]),
     "nmsamd_set_beam_waves": (C.c_int, [_P, C.c_int]),
     "nmsamd_brute_force": (C.c_int, [_P, _P, _I64, _I64, _I32, _P, _P, _P]),
     "nmsamd_last_phase_us": (C.c_int, [_P, _P]),
@@ -249,6 +250,12 @@ class FloatIndex:
         step's distances and rebuilds half of the sorted array) or 0 = chosen
         from the LDS footprint.  Results do not depend on it."""
         _check(load_library().nmsamd_set_beam_waves(self._h, int(waves)))
+
+    def setBeamGather(self, gather: int = 0):
+        """Vector loads in flight in the one-wave beam kernels: 1 = plain (two
+        waves per SIMD), 2 = wide (one wave per SIMD, every load of a round in
+        flight), 0 = chosen from the LDS footprint.  Results do not depend on it."""
+        _check(load_library().nmsamd_set_beam_gather(self._h, int(gather)))
 
     def collectStats(self, on: bool = True):
         _check(load_library().nmsamd_collect_stats(self._h, int(bool(on))))

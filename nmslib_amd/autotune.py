@@ -1,15 +1,26 @@
 """Picks the search form for a workload by measurement.
 
-The knobs of the device search are the beam width (expansions per step:
-0 = the reference's exact V1Merge, 1..4 = beam mode; with the gathers of a
-step in flight together when the LDS footprint already holds residency to
-about a wave per SIMD) and the vector storage used while traversing (FP32,
-BF16, FP8-e4m3 with a per-vector scale, each followed by an exact FP32 rerank
-of the best 2k candidates).  ``autotune`` runs a calibration batch -- with its
-own per-query k and ef -- in every candidate form, computes each form's
-recall@k against the device brute force, and keeps the fastest form whose
-recall is within ``recall_slack`` of the exact form's.  Choices are cached
-per (dimension, ef bucket) and applied to the index.
+The knobs of the device search (none changes which results a form returns,
+except storage and rerank, whose recall is measured):
+
+* beam width W: expansions per step, 0 = the reference's exact V1Merge,
+  1..16 = beam mode;
+* waves per query in beam mode: 1, or 2 (the second wave computes part of
+  each step's distances and rebuilds half of the sorted array);
+* gather width of the one-wave kernels: plain (two waves per SIMD, one FP32 /
+  two BF16 / four FP8 vectors per lane pair in flight) or wide (one wave per
+  SIMD, two / five / eight);
+* vector storage while traversing: FP32, BF16 or FP8-e4m3 with a per-vector
+  scale, the reduced ones followed by an exact FP32 rerank of the best
+  rerank * k candidates (the rerank width).
+
+``autotune`` runs a calibration batch -- with its own per-query k and ef --
+in a staged search: every storage x beam width with the kernel chosen from
+the LDS footprint, then, for the fastest forms within the recall slack, the
+waves per query, the gather width and (reduced precision) the rerank width.
+Recall@k is computed against the device brute force (exact), and the fastest
+form whose recall is within ``recall_slack`` of the exact FP32 form's is kept.
+Choices are cached per (dimension, ef bucket, max k) and applied to the index.
 """
 from __future__ import annotations
 
@@ -17,14 +28,22 @@ import time
 
 import numpy as np
 
-DEFAULT_FORMS = [
-    # (name, storage, rerank multiple of k, beam width)
-    ("fp32-exact", "fp32", 0, 0),
-    ("fp32-beam2", "fp32", 0, 2),
-    ("fp32-beam4", "fp32", 0, 4),
-    ("bf16-beam4-rerank", "bf16", 2, 4),
-    ("fp8-beam4-rerank", "fp8", 2, 4),
+# (name, storage, rerank multiple of k, beam width, waves per query, gather)
+Form = tuple
+
+STAGE1 = [
+    ("fp32-exact", "fp32", 0, 0, 0, 0),
+    ("fp32-beam2", "fp32", 0, 2, 0, 0),
+    ("fp32-beam4", "fp32", 0, 4, 0, 0),
+    ("fp32-beam8", "fp32", 0, 8, 0, 0),
+    ("fp32-beam16", "fp32", 0, 16, 0, 0),
+    ("bf16-beam4-rerank2", "bf16", 2, 4, 0, 0),
+    ("bf16-beam8-rerank2", "bf16", 2, 8, 0, 0),
+    ("bf16-beam16-rerank2", "bf16", 2, 16, 0, 0),
+    ("fp8-beam8-rerank2", "fp8", 2, 8, 0, 0),
 ]
+# kept for callers that pass the old five forms explicitly
+DEFAULT_FORMS = [f[:4] for f in STAGE1]
 
 _cache: dict = {}
 
@@ -42,43 +61,77 @@ def _ef_bucket(efs):
     return int(2 ** np.ceil(np.log2(max(2, int(np.max(efs))))))
 
 
+def _apply(index, form):
+    _, storage, rerank, beam, waves, gather = (tuple(form) + (0, 0))[:6]
+    index.setPrecision(storage, rerank)
+    index.setBeam(beam)
+    index.setBeamWaves(waves)
+    index.setBeamGather(gather)
+
+
+def _variants(form):
+    """Stage 2: the same form with explicit waves / gather / rerank width."""
+    name, storage, rerank, beam, _, _ = form
+    out = []
+    if beam > 0:
+        out.append((f"{name}-1wave-plain", storage, rerank, beam, 1, 1))
+        out.append((f"{name}-1wave-wide", storage, rerank, beam, 1, 2))
+        out.append((f"{name}-2waves", storage, rerank, beam, 2, 0))
+    if storage != "fp32":
+        for r in (1, 4):
+            out.append((f"{name.rsplit('-rerank', 1)[0]}-rerank{r}", storage, r, beam, 0, 0))
+    return out
+
+
 def autotune(index, queries, ks, efs, forms=None, recall_slack: float = 0.003, reps: int = 2,
-             apply: bool = True):
+             apply: bool = True, refine: int = 2):
+    """Measures the candidate forms on (queries, ks, efs) and keeps the
+    fastest within recall_slack of exact FP32.  `forms` replaces the first
+    stage's list; `refine` is how many of its best forms stage 2 varies."""
     queries = np.ascontiguousarray(queries, np.float32)
     ks = np.ascontiguousarray(ks, np.int32)
     efs = np.ascontiguousarray(efs, np.int32)
     key = (queries.shape[1], _ef_bucket(efs), int(ks.max()))
     if key in _cache and apply:
         best = _cache[key]
-        index.setPrecision(best["storage"], best["rerank"])
-        index.setBeam(best["beam"])
+        _apply(index, best["spec"])
         return best
-    forms = forms or DEFAULT_FORMS
+    stage1 = [tuple(f) + (0, 0)[:6 - len(f)] for f in (forms or STAGE1)]
     kmax = int(ks.max())
     gt_ids, gt_d, _ = index.bruteForce(queries, min(128, max(kmax, 10)))
     index.setQueryTimeParams({"algoType": "v1merge"})
-    report = []
-    for name, storage, rerank, beam in forms:
-        index.setPrecision(storage, rerank)
-        index.setBeam(beam)
+
+    def measure(form):
+        _apply(index, form)
         ids, _, _ = index.knnQueryBatchUneven(queries, ks, efs)  # warm: conversions, code objects
-        t = time.perf_counter()
+        el = float("inf")
         for _ in range(reps):
+            t = time.perf_counter()
             ids, _, _ = index.knnQueryBatchUneven(queries, ks, efs)
-        el = (time.perf_counter() - t) / reps
-        report.append({"form": name, "storage": storage, "rerank": rerank, "beam": beam,
-                       "seconds": el, "qps": len(queries) / el,
-                       "recall": _recall(ids, gt_ids, gt_d, ks)})
+            el = min(el, time.perf_counter() - t)
+        launch = index.lastLaunch()
+        return {"form": form[0], "spec": form, "storage": form[1], "rerank": form[2],
+                "beam": form[3], "waves": form[4], "gather": form[5], "seconds": el,
+                "qps": len(queries) / el, "recall": _recall(ids, gt_ids, gt_d, ks),
+                "kernel": launch.get("kernel"), "mode": launch.get("mode")}
+
+    report = [measure(f) for f in stage1]
     exact = next((r for r in report if r["beam"] == 0 and r["storage"] == "fp32"), report[0])
-    ok = [r for r in report if r["recall"] >= exact["recall"] - recall_slack]
+    floor = exact["recall"] - recall_slack
+    ok = sorted((r for r in report if r["recall"] >= floor), key=lambda r: -r["qps"])
+    seen = {r["form"] for r in report}
+    for r in ok[:refine]:
+        for v in _variants(r["spec"]):
+            if v[0] not in seen:
+                seen.add(v[0])
+                report.append(measure(v))
+    ok = [r for r in report if r["recall"] >= floor]
     best = dict(max(ok, key=lambda r: r["qps"]))
     best["candidates"] = report
     best["key"] = key
     if apply:
-        index.setPrecision(best["storage"], best["rerank"])
-        index.setBeam(best["beam"])
+        _apply(index, best["spec"])
         _cache[key] = best
     else:
-        index.setPrecision("fp32", 0)
-        index.setBeam(0)
+        _apply(index, ("fp32-exact", "fp32", 0, 0, 0, 0))
     return best

@@ -8,12 +8,17 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libnmslib_amd.so")
-SOURCES = ["search_f32.hip", "search_bf16.hip", "search_fp8.hip", "launch.hip",
-           "lowp_convert.hip", "bruteforce.hip", "bruteforce_exact.hip", "nmslib_amd.cpp", "hnsw_build.cpp"]
+# (source, object name, extra defines): search_inst.hip is compiled once per
+# (vector storage, kernel mode) slice so the slices build in parallel
+SOURCES = ([("search_inst.hip", f"search_st{st}_m{m}", [f"-DNMS_ST={st}", f"-DNMS_MODE={m}"])
+            for st in range(3) for m in range(4)]
+           + [(f, f, []) for f in ("launch.hip", "lowp_convert.hip", "bruteforce.hip",
+                                   "bruteforce_exact.hip", "nmslib_amd.cpp", "hnsw_build.cpp")])
 # host-only sources: the index builder (x86 SSE, no device code)
 HOST_SOURCES = {"hnsw_build.cpp"}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -42,30 +47,47 @@ def build(verbose: bool = False, incremental: bool = False) -> str:
     objs = []
     builddir = os.path.join(HERE, "_build")
     os.makedirs(builddir, exist_ok=True)
-    procs = []
-    for src in SOURCES:
-        obj = os.path.join(builddir, src + ".o")
+    jobs = []
+    for src, name, defines in SOURCES:
+        obj = os.path.join(builddir, name + ".o")
+        objs.append(obj)
         if incremental and not _stale(src, obj):
-            objs.append(obj)
             continue
-        cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC, *FLAGS, *defines, "-c", os.path.join(CSRC, src), "-o", obj]
         if src in HOST_SOURCES:
             cmd = [CXX, *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
         elif src.endswith(".cpp"):
             cmd = [HIPCC, *FLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+        jobs.append((name, cmd))
+    # at most one compiler per core (MAX_JOBS when set), largest slices first
+    limit = max(1, int(os.environ.get("MAX_JOBS", 0)) or os.cpu_count() or 4)
+    failed, running = [], []
+
+    def reap(block):
+        for name, p in list(running):
+            if block or p.poll() is not None:
+                out, _ = p.communicate()
+                running.remove((name, p))
+                text = out.decode(errors="replace")
+                if p.returncode != 0:
+                    failed.append(name)
+                    sys.stderr.write(text)
+                elif verbose and text.strip():
+                    sys.stderr.write(text)
+                if block:
+                    return
+
+    for name, cmd in jobs:
+        while len(running) >= limit:
+            reap(False)
+            if len(running) >= limit:
+                time.sleep(0.2)
         if verbose:
             print(" ".join(cmd), flush=True)
-        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
-        objs.append(obj)
-    failed = []
-    for src, p in procs:
-        out, _ = p.communicate()
-        text = out.decode(errors="replace")
-        if p.returncode != 0:
-            failed.append(src)
-            sys.stderr.write(text)
-        elif verbose and text.strip():
-            sys.stderr.write(text)
+        running.append((name, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                               stderr=subprocess.STDOUT)))
+    while running:
+        reap(True)
     if failed:
         raise RuntimeError(f"hipcc failed on {failed}")
     tmp = OUT + ".tmp"

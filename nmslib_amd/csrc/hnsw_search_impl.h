@@ -1303,8 +1303,15 @@ __device__ __forceinline__ void bitonic_sort(float (&k)[R], uint32_t (&id)[R], i
 // walk_pair() passes from the same command, so the barrier counts match.
 enum PairCmd : int { kCmdNop = 0, kCmdDist = 1, kCmdWalk = 2, kCmdRank = 3, kCmdExit = 4 };
 
+// The barrier between the two waves of a query.  Every LDS access this wave
+// issued completes before it arrives (s_waitcnt lgkmcnt(0), written out):
+// the release fence alone left a hand-off write (the second wave's admitted
+// count, ds_write then s_barrier on the loop's back edge) in flight at the
+// barrier in the generated code, and with other blocks' LDS traffic on the CU
+// the first wave then read the word, and the items, before the write landed.
 __device__ __forceinline__ void pair_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt and expcnt unconstrained
   __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }

@@ -9,16 +9,30 @@
 namespace nmsamd {
 
 using KernelFn = void (*)(SearchArgs, int);
-KernelFn pick_kernel_f32(int dim, int func, int mode);
-KernelFn pick_kernel_bf16(int dim, int func, int mode);
-KernelFn pick_kernel_fp8(int dim, int func, int mode);
+// the slices of search_inst.hip: pick_kernel_st<storage>_m<mode>
+#define NMS_DECL(S)                                    \
+  KernelFn pick_kernel_st##S##_m0(int dim, int func);  \
+  KernelFn pick_kernel_st##S##_m1(int dim, int func);  \
+  KernelFn pick_kernel_st##S##_m2(int dim, int func);  \
+  KernelFn pick_kernel_st##S##_m3(int dim, int func);
+NMS_DECL(0)
+NMS_DECL(1)
+NMS_DECL(2)
+#undef NMS_DECL
+
+static KernelFn pick_kernel(int storage, int dim, int func, int mode) {
+  using Pick = KernelFn (*)(int, int);
+  static const Pick table[3][4] = {
+      {pick_kernel_st0_m0, pick_kernel_st0_m1, pick_kernel_st0_m2, pick_kernel_st0_m3},
+      {pick_kernel_st1_m0, pick_kernel_st1_m1, pick_kernel_st1_m2, pick_kernel_st1_m3},
+      {pick_kernel_st2_m0, pick_kernel_st2_m1, pick_kernel_st2_m2, pick_kernel_st2_m3}};
+  if (storage < 0 || storage > 2 || mode < 0 || mode > 3) return nullptr;
+  return table[storage][mode](dim, func);
+}
 
 hipError_t launch_search(const SearchArgs& a, int blocks, int waves_per_block, int lds_wave_bytes,
                          int mode, hipStream_t stream) {
-  KernelFn fn = nullptr;
-  if (a.storage == kF32) fn = pick_kernel_f32(a.ix.dim, a.ix.func, mode);
-  else if (a.storage == kBF16) fn = pick_kernel_bf16(a.ix.dim, a.ix.func, mode);
-  else fn = pick_kernel_fp8(a.ix.dim, a.ix.func, mode);
+  KernelFn fn = pick_kernel(a.storage, a.ix.dim, a.ix.func, mode);
   if (!fn) return hipErrorInvalidValue;
   if (waves_per_block < 1 || waves_per_block > kWavesPerBlock) return hipErrorInvalidValue;
   // kModeBeamPair: one query per block, served by two waves on one LDS area

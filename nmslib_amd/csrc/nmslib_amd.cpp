@@ -107,6 +107,7 @@ struct nmsamd_index {
   int rerank_mult = 0;
   int beam = 0;  // 0: exact V1Merge; > 0: beam mode width
   int beam_waves = 0;  // waves per beam query: 0 = by LDS footprint, 1 or 2
+  int beam_gather = 0;  // one-wave beam kernel: 0 = by LDS footprint, 1 = plain, 2 = wide
   bool collect = false;
   // file header (hnsw.cc:779-788)
   bool loaded = false;
@@ -681,9 +682,14 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   const int wpb = pair ? 1 : (int)std::max<size_t>(1, std::min<size_t>(kWavesPerBlock, fit));
   const int per_cu = (int)std::max<size_t>(
       1, std::min<size_t>(pair ? 3 : 8, kLdsPerCu / ((size_t)lds_launch * wpb)));
-  const int mode = !beam ? kModeAll
-                         : (pair ? kModeBeamPair : (wpb * per_cu <= 4 ? kModeBeamWide : kModeBeam));
-  int64_t max_blocks = (int64_t)h->cu_count * per_cu;
+  // the one-wave kernel: wide (every vector load of a round in flight, the
+  // register file of one wave per SIMD) where the footprint already holds a
+  // CU to four waves, or as forced by setBeamGather; forced wide, the grid
+  // keeps to the four waves per CU it can hold
+  const bool wide = !pair && (h->beam_gather == 2 || (h->beam_gather == 0 && wpb * per_cu <= 4));
+  const int per_cu_run = wide ? std::max(1, std::min(per_cu, 4 / wpb)) : per_cu;
+  const int mode = !beam ? kModeAll : (pair ? kModeBeamPair : (wide ? kModeBeamWide : kModeBeam));
+  int64_t max_blocks = (int64_t)h->cu_count * per_cu_run;
   const int64_t heap_cap = any_old ? std::max<int64_t>(65536, 64 * (int64_t)max_ef) : 0;
   const int64_t vstride = ((int64_t)h->n + 1 + 255) / 256 * 256;
   // visited marks take n bytes per slot: keep them under ~16 GiB, or within
@@ -1290,6 +1296,15 @@ int nmsamd_set_beam(nmsamd_index* h, int beam) {
   });
 }
 
+int nmsamd_set_beam_gather(nmsamd_index* h, int gather) {
+  return guarded([&] {
+    if (!h) fail(NMSAMD_EINVAL, "null handle");
+    if (gather < 0 || gather > 2)
+      fail(NMSAMD_EINVAL, "beam gather must be 0 (by LDS footprint), 1 (plain) or 2 (wide)");
+    h->beam_gather = gather;
+  });
+}
+
 int nmsamd_set_beam_waves(nmsamd_index* h, int waves) {
   return guarded([&] {
     if (!h) fail(NMSAMD_EINVAL, "null handle");
@@ -1392,6 +1407,7 @@ struct nmsamd_server {
   // the captured graphs and every later launch follow these, whatever the
   // index's setters change afterwards
   int storage = kF32, rerank = 0, beam = 0;
+  int pair = 0;  // two waves per query (kModeBeamPair), from the LDS footprint
   DevIndex ix{};
   int slots_per_lane = 0;
   bool use_graph = false;
@@ -1440,8 +1456,9 @@ SearchArgs server_args(nmsamd_server* s, ServerLane& L) {
 void lane_launch(nmsamd_server* s, ServerLane& L) {
   SearchArgs a = server_args(s, L);
   hip_check(launch_fill_i32(L.work.p, 0, 2, L.stream), "fill");
-  // the server runs V1Merge, so beam batches always take the beam-only kernel
-  const int mode = a.beam > 0 ? (s->wide ? kModeBeamWide : kModeBeam) : kModeAll;
+  // the server runs V1Merge, so beam batches always take the beam-only kernels
+  const int mode = a.beam > 0 ? (s->pair ? kModeBeamPair : (s->wide ? kModeBeamWide : kModeBeam))
+                              : kModeAll;
   hip_check(launch_search(a, L.blocks, s->wpb, s->lds, mode, L.stream), "search kernel launch");
 }
 
@@ -1478,9 +1495,17 @@ int nmsamd_server_open(nmsamd_index* h, int lanes, int max_queries, int max_ef, 
     s->storage = h->storage;
     s->rerank = (h->storage != kF32 && h->rerank_mult > 0) ? h->rerank_mult : 0;
     s->beam = h->beam;
-    s->wpb = (int)std::max<size_t>(1, std::min<size_t>(kWavesPerBlock, kLdsPerCu / s->lds));
-    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, kLdsPerCu / ((size_t)s->lds * s->wpb)));
-    s->wide = (h->beam > 0 && s->wpb * per_cu <= 4) ? 1 : 0;
+    // the batch entry points' launch shape: two waves per query where the
+    // footprint limits a CU to three queries (one query per block, LDS
+    // padded so no fourth block fits), else one, wide where the footprint
+    // holds a CU to four waves
+    const size_t fit = kLdsPerCu / s->lds;
+    s->pair = (h->beam > 0 && h->beam_waves != 1 && (h->beam_waves == 2 || fit <= 3)) ? 1 : 0;
+    if (s->pair) s->lds = std::max(s->lds, (int)(kLdsPerCu / 4) + 16);
+    s->wpb = s->pair ? 1 : (int)std::max<size_t>(1, std::min<size_t>(kWavesPerBlock, fit));
+    const int per_cu = (int)std::max<size_t>(
+        1, std::min<size_t>(s->pair ? 3 : 8, kLdsPerCu / ((size_t)s->lds * s->wpb)));
+    s->wide = (h->beam > 0 && !s->pair && s->wpb * per_cu <= 4) ? 1 : 0;
     // the device's resident waves at this footprint, shared by the lanes
     const int64_t total_blocks = (int64_t)h->cu_count * per_cu;
     const int64_t blocks_per_lane = std::max<int64_t>(1, total_blocks / lanes);
@@ -1598,7 +1623,8 @@ int nmsamd_server_info(const nmsamd_server* s, int64_t* o) {
   return guarded([&] {
     if (!s || !o) fail(NMSAMD_EINVAL, "null argument");
     o[0] = (int64_t)s->lanes.size(); o[1] = s->max_queries; o[2] = s->max_ef; o[3] = s->max_k;
-    o[4] = s->slots_per_lane; o[5] = s->wpb; o[6] = s->use_graph ? 1 : 0; o[7] = s->wide;
+    o[4] = s->slots_per_lane; o[5] = s->wpb; o[6] = s->use_graph ? 1 : 0;
+    o[7] = s->pair ? 2 : s->wide;
   });
 }
 

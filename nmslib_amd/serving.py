@@ -65,8 +65,10 @@ class Server:
     def info(self):
         out = np.zeros(8, np.int64)
         _check(_lib().nmsamd_server_info(self._h, _ptr(out)))
+        # beam_kernel: 0 = one wave per query (plain), 1 = one wave (wide),
+        # 2 = two waves per query
         keys = ("lanes", "max_queries", "max_ef", "max_k", "slots_per_lane", "waves_per_block",
-                "graph", "wide")
+                "graph", "beam_kernel")
         return {k: int(v) for k, v in zip(keys, out)}
 
     def submit(self, lane: int, q: np.ndarray, ks: np.ndarray, efs: np.ndarray):

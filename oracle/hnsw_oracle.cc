@@ -411,8 +411,17 @@ struct BeamItem {
   bool operator<(const BeamItem& o) const { return key < o.key || (key == o.key && id < o.id); }
 };
 
+// One step of the beam model as the device's two-wave protocol sees it: the
+// array size before the step, the picks, the fresh neighbours, the admitted
+// items, and for each merge pass of at most 256 items (the device's pass
+// size) the position its smallest item takes.
+struct BeamStep {
+  int32_t n, nsel, ncand, m, p0[2];
+};
+
 void search_beam_one(const Index& ix, float* q, size_t k, size_t ef, int beam, Visited& vis,
-                     std::vector<BeamItem>& A, ora_stats& st) {
+                     std::vector<BeamItem>& A, ora_stats& st,
+                     std::vector<BeamStep>* trace = nullptr) {
   A.clear();
   if (ix.n == 0 || k == 0) return;
   const int64_t d = ix.dim;
@@ -458,6 +467,26 @@ void search_beam_one(const Index& ix, float* q, size_t k, size_t ef, int beam, V
       if (dd < topk || nsnap < ef) add.push_back({dd, t, false});
     }
     std::sort(add.begin(), add.end());
+    if (trace) {
+      BeamStep bs{(int32_t)A.size(), (int32_t)picks.size(), (int32_t)fresh.size(),
+                  (int32_t)add.size(), {-1, -1}};
+      // passes of 256 merged one after another: the first item of pass p lands
+      // after the A items below it and the earlier passes' items below it
+      // (all of them: the passes are in order), truncated at C in between
+      size_t n = A.size();
+      std::vector<BeamItem> cur(A.begin(), A.end());
+      for (size_t b = 0, p = 0; b < add.size() && p < 2; b += 256, ++p) {
+        const size_t cnt = std::min<size_t>(256, add.size() - b);
+        bs.p0[p] = (int32_t)(std::lower_bound(cur.begin(), cur.end(), add[b]) - cur.begin());
+        std::vector<BeamItem> mg(cur.size() + cnt);
+        std::merge(cur.begin(), cur.end(), add.begin() + b, add.begin() + b + cnt, mg.begin());
+        if (mg.size() > C) mg.resize(C);
+        cur.swap(mg);
+        n = cur.size();
+      }
+      (void)n;
+      trace->push_back(bs);
+    }
     std::vector<BeamItem> merged(A.size() + add.size());
     std::merge(A.begin(), A.end(), add.begin(), add.end(), merged.begin());
     if (merged.size() > C) merged.resize(C);
@@ -563,6 +592,28 @@ int ora_search(void* h, const float* q, int64_t nq, int k, int ef, int algo, int
       }
     }
     return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+// The beam model's steps for one query (BeamStep records, 6 ints each, up to
+// cap steps); returns the number of steps, or -1 on error.
+int64_t ora_beam_trace(void* h, const float* q, int k, int ef, int beam, int32_t* out,
+                       int64_t cap) {
+  try {
+    const Index& ix = *static_cast<Index*>(h);
+    if (k < 0 || ef < 0 || beam < 1) throw std::invalid_argument("k, ef >= 0 and beam >= 1 required");
+    Visited vis(ix.n);
+    std::vector<float> qv(q, q + ix.dim);
+    std::vector<BeamItem> A;
+    ora_stats st{0, 0, 0, 0};
+    std::vector<BeamStep> tr;
+    search_beam_one(ix, qv.data(), (size_t)k, (size_t)ef, beam, vis, A, st, &tr);
+    for (size_t i = 0; i < tr.size() && (int64_t)i < cap; ++i)
+      std::memcpy(out + 6 * i, &tr[i], sizeof(BeamStep));
+    return (int64_t)tr.size();
   } catch (const std::exception& e) {
     g_err = e.what();
     return -1;

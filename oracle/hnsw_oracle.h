@@ -63,6 +63,14 @@ int ora_search(void* h, const float* q, int64_t nq, int k, int ef, int algo, int
 int ora_search_beam(void* h, const float* q, int64_t nq, int k, int ef, int beam, int num_threads,
                     int32_t* ids, float* dists, int32_t* counts, ora_stats* stats);
 
+/* The beam model's steps for one query: per step 6 ints -- array size before
+ * the step, picks, fresh neighbours, admitted items, and the position the
+ * smallest item of each of the (at most two) 256-item merge passes takes
+ * (-1: no such pass).  Fills up to cap steps; returns their number (-1 on
+ * error). */
+int64_t ora_beam_trace(void* h, const float* q, int k, int ef, int beam, int32_t* out,
+                       int64_t cap);
+
 float ora_dist(int func, const float* q, const float* x, int64_t dim);
 void ora_normalize(float* v, int64_t dim);
 

@@ -54,6 +54,8 @@ def lib():
         l.ora_search.argtypes = [P, P, I64, I32, I32, I32, I32, P, P, P, P]
         l.ora_search_beam.restype = I32
         l.ora_search_beam.argtypes = [P, P, I64, I32, I32, I32, I32, P, P, P, P]
+        l.ora_beam_trace.restype = C.c_int64
+        l.ora_beam_trace.argtypes = [P, P, I32, I32, I32, P, I64]
         l.ora_dist.restype = C.c_float
         l.ora_dist.argtypes = [I32, P, P, I64]
         l.ora_normalize.argtypes = [P, I64]
@@ -117,6 +119,17 @@ class OracleIndex:
         if with_stats:
             return ids, dists, counts, {f: getattr(st, f) for f, _ in Stats._fields_}
         return ids, dists, counts
+
+    def beam_trace(self, q, k: int, ef: int, beam: int) -> np.ndarray:
+        """Per-step records of the beam model for one query: columns n,
+        picks, fresh, admitted, first-item position of merge pass 0 and 1."""
+        q = np.ascontiguousarray(q, dtype=np.float32).reshape(self.dim)
+        cap = 1 << 16
+        out = np.empty((cap, 6), np.int32)
+        n = lib().ora_beam_trace(self._h, _p(q), k, ef, beam, _p(out), cap)
+        if n < 0:
+            raise RuntimeError(lib().ora_last_error().decode())
+        return out[:min(n, cap)].copy()
 
     def level0_block(self) -> np.ndarray:
         ptr = lib().ora_level0(self._h)

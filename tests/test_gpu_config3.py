@@ -100,3 +100,57 @@ def test_cfg3_uneven_beam16(cfg3, storage, rerank):
         assert counts[i] == oc[0], i
         np.testing.assert_array_equal(ids[i, :k], oi[0], err_msg=f"query {i} k={k} ef={ef}")
         np.testing.assert_array_equal(dists[i, :k].view(np.uint32), od[0].view(np.uint32))
+
+
+def test_cfg3_autotuner_pick_keeps_pace_with_the_hand_set_form(cfg3):
+    # the autotuner (storage x beam width, then waves per query, gather width
+    # and rerank width for the best forms) calibrated on a quarter of the
+    # batch: on the whole batch its pick keeps the exact search's recall
+    # (within the slack) and is at least as fast as the hand-set BF16 beam 8
+    # + rerank x2 form, one of the forms it weighs
+    from nmslib_amd import autotune
+    path, idx, q, ks, efs, gt = cfg3
+    cal = slice(0, len(q), 4)
+    best = autotune.autotune(idx, q[cal], ks[cal], efs[cal], reps=2)
+    print({c["form"]: (round(c["qps"]), round(c["recall"], 4)) for c in best["candidates"]},
+          flush=True)
+    pick = _timed_uneven(idx, q, ks, efs)
+    autotune._apply(idx, ("bf16-beam8-rerank2", "bf16", 2, 8, 0, 0))
+    hand = _timed_uneven(idx, q, ks, efs)
+    autotune._apply(idx, ("fp32-exact", "fp32", 0, 0, 0, 0))
+    exact = _timed_uneven(idx, q, ks, efs)
+
+    def mean_recall(ids):
+        r = _recall(ids, ks, gt)
+        return sum(r[kk] * np.sum(ks == kk) for kk in r) / len(ks)
+
+    r_pick, r_hand, r_exact = mean_recall(pick[0]), mean_recall(hand[0]), mean_recall(exact[0])
+    print(f"cfg3 autotuned {best['form']}: kernel {pick[3]:.1f} ms, recall {r_pick:.4f}; "
+          f"bf16 beam 8: {hand[3]:.1f} ms, {r_hand:.4f}; fp32 exact: {exact[3]:.1f} ms, "
+          f"{r_exact:.4f}", flush=True)
+    assert r_pick >= r_exact - 0.006
+    if r_hand >= r_exact - 0.003:
+        assert pick[3] <= hand[3] * 1.05
+
+
+
+@pytest.mark.parametrize("storage,rerank", [("fp32", 0), ("bf16", 2)])
+def test_cfg3_two_waves_equal_one_wave(cfg3, storage, rerank):
+    # the whole uneven batch with two waves per query (768 blocks, three per
+    # CU, dimension 100: the instantiations whose hand-off barrier lacked its
+    # LDS wait) returns exactly the one-wave kernel's rows
+    path, idx, q, ks, efs, gt = cfg3
+    idx.setPrecision(storage, rerank)
+    idx.setBeam(8)
+    idx.setQueryTimeParams({"algoType": "v1merge"})
+    idx.setBeamWaves(1)
+    one = idx.knnQueryBatchUneven(q, ks, efs)
+    idx.setBeamWaves(2)
+    two = idx.knnQueryBatchUneven(q, ks, efs)
+    launch = idx.lastLaunch()
+    idx.setBeamWaves(0)
+    idx.setBeam(0)
+    idx.setPrecision("fp32", 0)
+    assert launch["mode"] == "beam (two waves)" and launch["blocks"] > 512, launch
+    for a, b in zip(one, two):
+        np.testing.assert_array_equal(a, b)

@@ -211,12 +211,6 @@ def test_gpu_beam_mode_equals_model(name, beam, waves):
     # chunks, k > ef, and steps of up to 512 entries; one wave per query, and
     # two (the second wave computing part of the distances and rebuilding
     # half of the array)
-    if (name, waves) == ("cfg1_l2", 2):
-        # three variants of the two-wave kernel (adjacency rows shared between
-        # the waves; a sort split between them) faulted the GPU on this index
-        # (1,000 queries at four per CU) with the cause not found; the variants
-        # are not built, and these cases stay off the device until then
-        pytest.skip("cases kept off the GPU after faults of two-wave variants")
     m = F.META[name]
     idx = load(name, F.index_path(name), m["space"])
     oracle = O.OracleIndex(F.index_path(name))

@@ -55,3 +55,28 @@ def test_fullsize_server_mix_equals_alone(big, rate, storage, rerank, beam):
         np.testing.assert_array_equal(gd, dists[:, :r.k], err_msg=f"request {r.rid}")
     idx.setPrecision("fp32", 0)
     idx.setBeam(0)
+
+
+def test_fullsize_server_two_wave_lanes_at_the_recall_point(big):
+    # every request at efSearch 4736 (recall@10 >= 0.95): the lanes take the
+    # two-wave kernel, as the batch entry point does at that footprint
+    idx, pool = big
+    idx.setPrecision("bf16", 2)
+    idx.setBeam(16)
+    srv = serving.Server(idx, lanes=4, max_queries=512, max_ef=4736, max_k=10, use_graph=True)
+    assert srv.info()["beam_kernel"] == 2
+    reqs = serving.make_requests(len(pool), rate_qps=20_000, duration_s=0.3, seed=11,
+                                 size_range=(1, 128), ks=(10,), ef_range=(4736, 4736))
+    st = serving.run_poisson(srv, pool, reqs, keep_results=True)
+    srv.close()
+    print(f"bf16 beam 16, ef 4736, two-wave lanes, offered 20k q/s: {st}", flush=True)
+    for r in reqs[::max(1, len(reqs) // 12)]:
+        q = pool[r.rows]
+        ids, dists, counts = idx.knnQueryBatchUneven(q, np.full(len(q), r.k, np.int32),
+                                                     np.full(len(q), r.ef, np.int32))
+        gi, gd, gc = r.result
+        np.testing.assert_array_equal(gc, counts, err_msg=f"request {r.rid}")
+        np.testing.assert_array_equal(gi, ids[:, :r.k], err_msg=f"request {r.rid}")
+        np.testing.assert_array_equal(gd, dists[:, :r.k], err_msg=f"request {r.rid}")
+    idx.setPrecision("fp32", 0)
+    idx.setBeam(0)

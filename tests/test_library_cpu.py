@@ -134,3 +134,11 @@ def test_load_rejects_upper_neighbour_without_that_level(tmp_path):
     with pytest.raises(RuntimeError, match="has no list at level 1"):
         idx.loadIndex(str(bad))
     idx.loadIndex(path)  # the untouched file loads
+
+
+def test_beam_gather_setter_validates():
+    idx = nmslib.init(space="l2")
+    for g in (0, 1, 2):
+        idx.setBeamGather(g)
+    with pytest.raises(ValueError, match="beam gather"):
+        idx.setBeamGather(3)
