@@ -538,7 +538,7 @@ def main():
                          args.serve_batch, ef_fixed=head["efSearch"], max_ef=head["efSearch"],
                          ks=(10,)) if hi_rates else []
         alone = single_query_latency(idx, q, serve_forms, head["efSearch"])
-        serve = {"config5_mix": mix, "at_recall_0_95": hi, "single_query_latency": alone}
+        serve = {"poisson_mix": mix, "at_recall_0_95": hi, "single_query_latency": alone}
 
     cpu = None
     if not args.no_cpu:

@@ -35,7 +35,7 @@ from refdrv import RefIndex  # noqa: E402
 # name: (space, generator, n, dim, seed, nq, qseed, index params)
 FIXTURES = {
     "u128_l2": ("l2", "uniform", 2000, 128, 11, 200, 12, "M=16,efConstruction=200"),
-    "cfg1_l2": ("l2", "uniform", 10000, 128, 0, 1000, 1, "M=16,efConstruction=200"),
+    "u10k_l2": ("l2", "uniform", 10000, 128, 0, 1000, 1, "M=16,efConstruction=200"),
     "hn128_l2": ("l2", "halfnormal_u8", 3000, 128, 13, 200, 14, "M=16,efConstruction=500"),
     "u100_l2": ("l2", "uniform", 1500, 100, 15, 200, 16, "M=8,efConstruction=100"),
     "dec100_cos": ("cosinesimil", "decaying_normal", 2000, 100, 17, 200, 18,
@@ -77,7 +77,7 @@ def main(only=None):
             indexpack.save_pack(pk, os.path.join(HERE, f"idx_{name}.npz"))
             loaded = RefIndex.load(space, path, dim)
         out = {"queries": q}
-        runs = RUNS if name != "cfg1_l2" else [(100, 10, "hybrid"), (10, 10, "hybrid"),
+        runs = RUNS if name != "u10k_l2" else [(100, 10, "hybrid"), (10, 10, "hybrid"),
                                               (32, 10, "hybrid"), (200, 10, "hybrid")]
         for ef, k, algo in runs:
             loaded.set_query_params(f"efSearch={ef},algoType={algo}")

@@ -87,7 +87,7 @@ def test_build_is_byte_identical_to_reference(name):
 
 
 # the two runs of make_goldens.py that wrote the query fixtures, in build order
-GOLDEN_RUNS = [["u128_l2", "cfg1_l2", "hn128_l2", "u100_l2", "dec100_cos", "gmm96_nd", "tiny_l2"],
+GOLDEN_RUNS = [["u128_l2", "u10k_l2", "hn128_l2", "u100_l2", "dec100_cos", "gmm96_nd", "tiny_l2"],
                ["u67_l1", "u64_linf"]]
 
 

@@ -20,7 +20,7 @@ def _stored_vectors(name):
     return oracle, blk[:, 16:16 + 4 * oracle.dim].copy().view(np.float32)
 
 
-@pytest.mark.parametrize("name", ["u128_l2", "u100_l2", "hn128_l2", "cfg1_l2", "tiny_l2",
+@pytest.mark.parametrize("name", ["u128_l2", "u100_l2", "hn128_l2", "u10k_l2", "tiny_l2",
                                   "gmm96_nd"])
 def test_seq_search_equals_reference_goldens(name):
     # l2 (sqrt of the SSE-order sum) and negdotprod: ids and FP32 distance

@@ -25,12 +25,12 @@ SAMPLE = slice(0, 8192, 128)  # 64 queries of the batch
 
 
 @pytest.fixture(scope="module")
-def cfg3():
+def decay2m():
     if not os.path.exists(PACK):
         pytest.skip("2M cosine workload pack not present")
     from nmslib_amd import datasets as D
     from nmslib_amd import indexpack
-    path = os.path.join(tempfile.mkdtemp(prefix="nmsamd_cfg3_"), "decay2m.hnsw")
+    path = os.path.join(tempfile.mkdtemp(prefix="nmsamd_decay2m_"), "decay2m.hnsw")
     indexpack.unpack(indexpack.load_pack(PACK), D.base("decay2m"), path)
     q = D.queries("decay2m")
     rng = np.random.default_rng(8)
@@ -59,14 +59,14 @@ def _timed_uneven(idx, q, ks, efs):
     return ids, dists, counts, idx.lastStats()["kernel_ms"], wall
 
 
-def test_cfg3_uneven_exact_equals_oracle(cfg3):
-    path, idx, q, ks, efs, gt = cfg3
+def test_decay2m_uneven_exact_equals_oracle(decay2m):
+    path, idx, q, ks, efs, gt = decay2m
     idx.setBeam(0)
     idx.setPrecision("fp32", 0)
     idx.setQueryTimeParams({"algoType": "v1merge"})
     ids, dists, counts, ms, wall = _timed_uneven(idx, q, ks, efs)
     rec = _recall(ids, ks, gt)
-    print(f"cfg3 exact uneven: {len(q) / ms * 1e3:.0f} q/s (kernel {ms:.1f} ms, wall {wall * 1e3:.1f} ms),"
+    print(f"decay2m exact uneven: {len(q) / ms * 1e3:.0f} q/s (kernel {ms:.1f} ms, wall {wall * 1e3:.1f} ms),"
           f" recall by k {rec}", flush=True)
     oracle = O.OracleIndex(path)
     for i in range(len(q))[SAMPLE]:
@@ -78,8 +78,8 @@ def test_cfg3_uneven_exact_equals_oracle(cfg3):
 
 
 @pytest.mark.parametrize("storage,rerank", [("fp32", 0), ("bf16", 2)])
-def test_cfg3_uneven_beam16(cfg3, storage, rerank):
-    path, idx, q, ks, efs, gt = cfg3
+def test_decay2m_uneven_beam16(decay2m, storage, rerank):
+    path, idx, q, ks, efs, gt = decay2m
     idx.setBeam(16)
     idx.setPrecision(storage, rerank)
     idx.setQueryTimeParams({"algoType": "v1merge"})
@@ -88,7 +88,7 @@ def test_cfg3_uneven_beam16(cfg3, storage, rerank):
     launch = idx.lastLaunch()
     idx.setBeam(0)
     idx.setPrecision("fp32", 0)
-    print(f"cfg3 {storage} beam16 rerank x{rerank} uneven: {len(q) / ms * 1e3:.0f} q/s "
+    print(f"decay2m {storage} beam16 rerank x{rerank} uneven: {len(q) / ms * 1e3:.0f} q/s "
           f"(kernel {ms:.1f} ms, wall {wall * 1e3:.1f} ms), recall by k {rec}, {launch}", flush=True)
     if storage != "fp32":
         return
@@ -102,14 +102,14 @@ def test_cfg3_uneven_beam16(cfg3, storage, rerank):
         np.testing.assert_array_equal(dists[i, :k].view(np.uint32), od[0].view(np.uint32))
 
 
-def test_cfg3_autotuner_pick_keeps_pace_with_the_hand_set_form(cfg3):
+def test_decay2m_autotuner_pick_keeps_pace_with_the_hand_set_form(decay2m):
     # the autotuner (storage x beam width, then waves per query, gather width
     # and rerank width for the best forms) calibrated on a quarter of the
     # batch: on the whole batch its pick keeps the exact search's recall
     # (within the slack) and is at least as fast as the hand-set BF16 beam 8
     # + rerank x2 form, one of the forms it weighs
     from nmslib_amd import autotune
-    path, idx, q, ks, efs, gt = cfg3
+    path, idx, q, ks, efs, gt = decay2m
     cal = slice(0, len(q), 4)
     best = autotune.autotune(idx, q[cal], ks[cal], efs[cal], reps=2)
     print({c["form"]: (round(c["qps"]), round(c["recall"], 4)) for c in best["candidates"]},
@@ -125,7 +125,7 @@ def test_cfg3_autotuner_pick_keeps_pace_with_the_hand_set_form(cfg3):
         return sum(r[kk] * np.sum(ks == kk) for kk in r) / len(ks)
 
     r_pick, r_hand, r_exact = mean_recall(pick[0]), mean_recall(hand[0]), mean_recall(exact[0])
-    print(f"cfg3 autotuned {best['form']}: kernel {pick[3]:.1f} ms, recall {r_pick:.4f}; "
+    print(f"decay2m autotuned {best['form']}: kernel {pick[3]:.1f} ms, recall {r_pick:.4f}; "
           f"bf16 beam 8: {hand[3]:.1f} ms, {r_hand:.4f}; fp32 exact: {exact[3]:.1f} ms, "
           f"{r_exact:.4f}", flush=True)
     assert r_pick >= r_exact - 0.006
@@ -135,11 +135,11 @@ def test_cfg3_autotuner_pick_keeps_pace_with_the_hand_set_form(cfg3):
 
 
 @pytest.mark.parametrize("storage,rerank", [("fp32", 0), ("bf16", 2)])
-def test_cfg3_two_waves_equal_one_wave(cfg3, storage, rerank):
+def test_decay2m_two_waves_equal_one_wave(decay2m, storage, rerank):
     # the whole uneven batch with two waves per query (768 blocks, three per
     # CU, dimension 100: the instantiations whose hand-off barrier lacked its
     # LDS wait) returns exactly the one-wave kernel's rows
-    path, idx, q, ks, efs, gt = cfg3
+    path, idx, q, ks, efs, gt = decay2m
     idx.setPrecision(storage, rerank)
     idx.setBeam(8)
     idx.setQueryTimeParams({"algoType": "v1merge"})

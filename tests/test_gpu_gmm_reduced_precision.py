@@ -30,7 +30,7 @@ SAMPLE = slice(0, 10_000, 625)  # 16 queries
 
 
 @pytest.fixture(scope="module")
-def cfg4():
+def gmm():
     require_gpu()
     import nmslib_amd as nmslib
     from nmslib_amd import datasets as D
@@ -42,10 +42,10 @@ def cfg4():
         # rows of the file are the generator's rows and labels are row numbers
         assert np.array_equal(pk["ext_id"], np.arange(len(pk["ext_id"]), dtype=np.int32))
         x = D.base(name)
-        path = os.path.join(tempfile.mkdtemp(prefix="nmsamd_cfg4_"), f"{name}.hnsw")
+        path = os.path.join(tempfile.mkdtemp(prefix="nmsamd_gmm_"), f"{name}.hnsw")
         indexpack.unpack(pk, x, path)
         del pk
-        print(f"cfg4: {name} index written from the reference-built pack", flush=True)
+        print(f"gmm: {name} index written from the reference-built pack", flush=True)
     else:
         name = "gmm2m"
         x = D.base(name)
@@ -55,19 +55,19 @@ def cfg4():
         t = time.time()
         b.createIndex({"M": 24, "efConstruction": 200, "indexThreadQty": threads},
                       print_progress=True)
-        path = os.path.join(tempfile.mkdtemp(prefix="nmsamd_cfg4_"), f"{name}.hnsw")
+        path = os.path.join(tempfile.mkdtemp(prefix="nmsamd_gmm_"), f"{name}.hnsw")
         b.saveIndex(path)
         del b
-        print(f"cfg4: {name} index ({len(x):,} x {x.shape[1]}) built by createIndex on {threads} "
+        print(f"gmm: {name} index ({len(x):,} x {x.shape[1]}) built by createIndex on {threads} "
               f"host threads in {time.time() - t:.0f} s", flush=True)
     q = D.queries(name)
     idx = load(name, path, "negdotprod")
-    print("cfg4: index loaded", flush=True)
+    print("gmm: index loaded", flush=True)
     gt, gd, _ = idx.bruteForce(q, 100)
     # the device brute force against the FP64 host oracle on the sample
     hi, hd = O.brute_force(x, q[SAMPLE], 10, "negdotprod")
     assert recall_tie_aware(gt[SAMPLE], hi, hd, 10, rel=1e-6) == 1.0
-    print("cfg4: ground truth ready", flush=True)
+    print("gmm: ground truth ready", flush=True)
     yield path, idx, x, q, gt, gd
     os.remove(path)
 
@@ -93,11 +93,11 @@ def _run(idx, q, ef, beam, storage, rerank):
     return ids, dists, counts, ms
 
 
-def test_cfg4_exact_equals_oracle(cfg4):
-    path, idx, x, q, gt, gd = cfg4
+def test_gmm_exact_equals_oracle(gmm):
+    path, idx, x, q, gt, gd = gmm
     ids, dists, counts, ms = _run(idx, q, 100, 0, "fp32", 0)
     rec = recall_tie_aware(ids, gt, gd, 10)
-    print(f"cfg4 fp32 exact ef=100: {len(q) / ms * 1e3:.0f} q/s, recall@10 {rec:.4f}", flush=True)
+    print(f"gmm fp32 exact ef=100: {len(q) / ms * 1e3:.0f} q/s, recall@10 {rec:.4f}", flush=True)
     oracle = O.OracleIndex(path)
     oi, od, oc = oracle.search(q[SAMPLE], 10, 100, "v1merge")
     np.testing.assert_array_equal(counts[SAMPLE], oc)
@@ -106,8 +106,8 @@ def test_cfg4_exact_equals_oracle(cfg4):
 
 
 @pytest.mark.parametrize("storage", ["bf16", "fp8"])
-def test_cfg4_reduced_precision_with_rerank(cfg4, storage):
-    _, idx, x, q, gt, gd = cfg4
+def test_gmm_reduced_precision_with_rerank(gmm, storage):
+    _, idx, x, q, gt, gd = gmm
     # BF16 keeps FP32's recall; FP8 e4m3 (3 mantissa bits) moves a distance
     # by up to ~1.5e-2 here, comparable to the gaps between neighbours of this
     # mixture, so its traversal loses recall that the rerank only partly wins
@@ -126,7 +126,7 @@ def test_cfg4_reduced_precision_with_rerank(cfg4, storage):
         lids, ldists, _, _ = _run(idx, q[SAMPLE], ef, beam, storage, 0)
         l64 = _fp64_dist(x, q[SAMPLE], lids)
         err_tr = np.abs(ldists.astype(np.float64) - l64)
-        print(f"cfg4 {storage} beam={beam} ef={ef} rerank x{mult}: {len(q) / ms * 1e3:.0f} q/s "
+        print(f"gmm {storage} beam={beam} ef={ef} rerank x{mult}: {len(q) / ms * 1e3:.0f} q/s "
               f"(fp32 {len(q) / ex_ms * 1e3:.0f} q/s), recall@10 {r_lo:.4f} (fp32 {r_ex:.4f}); "
               f"|d - d64| after rerank max {err_rr:.2e}, traversal mean {err_tr.mean():.2e} "
               f"max {err_tr.max():.2e}", flush=True)

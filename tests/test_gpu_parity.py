@@ -51,15 +51,15 @@ def test_gpu_matches_oracle(name, ef, k):
     np.testing.assert_array_equal(counts, oc)
 
 
-def test_gpu_config1_full_batch():
+def test_gpu_uniform10k_full_batch():
     # 10,000 x 128 uniform, l2, M=16, efC=200; 1,000 queries, k=10, ef=100
-    name = "cfg1_l2"
+    name = "u10k_l2"
     idx = load(name, F.index_path(name), "l2", 100)
     res = F.results(name)
     idx.collectStats(True)
     ids, dists, counts = idx.knnQueryBatchArrays(res["queries"], 10)
     assert_same(ids, dists, counts, res["ef100_k10_hybrid_ids"], res["ef100_k10_hybrid_dists"],
-                res["ef100_k10_hybrid_counts"], "cfg1")
+                res["ef100_k10_hybrid_counts"], "u10k_l2")
     st = idx.lastStats()
     _, _, _, ost = O.OracleIndex(F.index_path(name)).search(res["queries"], 10, 100,
                                                             with_stats=True)
@@ -200,7 +200,7 @@ def test_gpu_beam_mode_quality(name, beam):
     assert r_beam >= r_exact - 0.003, (r_beam, r_exact)
 
 
-@pytest.mark.parametrize("name", ["u128_l2", "hn128_l2", "dec100_cos", "gmm96_nd", "cfg1_l2",
+@pytest.mark.parametrize("name", ["u128_l2", "hn128_l2", "dec100_cos", "gmm96_nd", "u10k_l2",
                                   "u67_l1", "u64_linf"])
 @pytest.mark.parametrize("beam", [1, 2, 5, 8, 16])
 @pytest.mark.parametrize("waves", [1, 2])

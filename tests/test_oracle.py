@@ -181,11 +181,11 @@ def test_tiny_graph_with_ef_at_least_n_is_exact():
     np.testing.assert_allclose(dists, gd, rtol=1e-6)
 
 
-def test_recall_of_config1_grows_with_ef():
+def test_recall_of_uniform10k_grows_with_ef():
     # Uniform [0,1)^128 has a high intrinsic dimension, so recall stays below
     # the final128 ranges of test_integr.cc:86-89; what holds is that the
     # reference's recall grows with ef and that ef >= N would be exact.
-    name = "cfg1_l2"
+    name = "u10k_l2"
     res = F.results(name)
     x = F.base_vectors(name)
     gt, gd = O.brute_force(x, res["queries"], 100, "l2")

@@ -103,7 +103,7 @@ def replay(trace, W, E, C, ef, storage, regions):
 @pytest.mark.parametrize("storage", ["fp32", "bf16"])
 @pytest.mark.parametrize("beam", [1, 2, 5, 8, 16])
 def test_pair_protocol_replay_on_the_10k_fixture(beam, storage):
-    name = "cfg1_l2"
+    name = "u10k_l2"
     path = F.index_path(name)
     o = O.OracleIndex(path)
     q = F.results(name)["queries"]

@@ -26,7 +26,7 @@ def _run(idx, entry, q, k):
 
 @pytest.mark.parametrize("name,params,beam,k", [
     ("u128_l2", {"efSearch": 100, "algoType": "v1merge"}, 0, 10),
-    ("cfg1_l2", {"efSearch": 64, "algoType": "old"}, 0, 10),
+    ("u10k_l2", {"efSearch": 64, "algoType": "old"}, 0, 10),
     ("dec100_cos", {"efSearch": 300, "algoType": "v1merge"}, 8, 100),
 ])
 def test_no_alloc_entry_equals_device_entry(name, params, beam, k):
