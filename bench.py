@@ -256,7 +256,7 @@ def measure(idx, q, gids, gd, form, args, dev, log_prefix):
     st = idx.lastStats()
     launch = idx.lastLaunch()
     lat = idx.lastQueryTimesUs(nq)
-    phases = idx.lastPhaseUs() if beam else None
+    phases = idx.lastPhaseUs() if args.algo != "old" else None
     idx.collectStats(False)
     ids_np = dids.cpu().numpy()
     dists_np = dd.cpu().numpy()

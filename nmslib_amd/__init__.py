@@ -270,12 +270,17 @@ class FloatIndex:
         return d
 
     def lastPhaseUs(self):
-        """Beam mode with collectStats(True): device time per step phase, us
-        summed over the queries of the last search."""
+        """With collectStats(True): device time per phase of a step (beam
+        mode) or of an expansion (exact V1Merge), us summed over the queries
+        of the last search."""
         out = np.zeros(8, np.float64)
         _check(load_library().nmsamd_last_phase_us(self._h, _ptr(out)))
-        keys = ("pick", "adjacency_visited", "distances", "filter", "sort_predict",
-                "rank_search", "rebuild", "advance")
+        if self.lastLaunch()["mode"] == "v1merge/old":
+            keys = ("pick_adjacency", "visited", "distances", "admit_predict", "sort",
+                    "rank_search", "merge", "advance")
+        else:
+            keys = ("pick", "adjacency_visited", "distances", "filter", "sort_predict",
+                    "rank_search", "rebuild", "advance")
         return {k: float(v) for k, v in zip(keys, out)}
 
     def lastLaunch(self):

@@ -104,7 +104,7 @@ struct SearchArgs {
   int32_t* old_heap_id;
   int64_t old_heap_cap;
   int* err;                  // bit 1: a SearchOld candidate heap overflowed; bit 2: the
-                             // search state held a node id outside the index
+                             // search state held a node id outside the index or expanded an item twice
 };
 
 // algoType (hnsw.cc:491-501); hybrid picks SearchOld for ef >= 1000 per query

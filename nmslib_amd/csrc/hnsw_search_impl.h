@@ -626,11 +626,13 @@ struct WideNV {  // vectors in flight per lane pair: about 128 registers of payl
 
 // Vectors per lane pair and round of dist_cands: PATH 0 = one (dist_node),
 // 1 = WideNV (a wave with the register file of one wave per SIMD), 2 = two
-// BF16 / four FP8 vectors (two waves per SIMD; FP32 keeps one).
+// BF16 / four FP8 vectors (two waves per SIMD; FP32 keeps one), 3 = one, its
+// loads all issued before any arithmetic (the exact search).
 template <int ST, int PATH>
 struct PathNV {
   static constexpr int value =
-      PATH == 1 ? WideNV<ST>::value : (PATH == 2 ? (ST == kF32 ? 0 : (ST == kBF16 ? 2 : 4)) : 0);
+      PATH == 1 ? WideNV<ST>::value
+                : (PATH == 2 ? (ST == kF32 ? 0 : (ST == kBF16 ? 2 : 4)) : (PATH == 3 ? 1 : 0));
 };
 
 // cdist[s] = distance of cand[s] for s < ncand (fallback: a valid node id used
@@ -650,6 +652,9 @@ __device__ __forceinline__ void dist_cands(const float* __restrict__ qs, const D
         const int id = checked_node(s < ncand ? cand[s] : fallback, (int)ix.n, err);
         vec_load<D, ST>(r[v], ix, id, h);
       }
+      // keep the scheduler from interleaving the loads with the arithmetic
+      // (it would otherwise wait on each pair of loads in turn)
+      if constexpr (WIDE == 3) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         const int s = b + 32 * v + (lane >> 1);
@@ -969,6 +974,15 @@ __device__ __forceinline__ void finish_query(const SearchArgs& a, const WaveLds&
 
 // ---------------------------------------------------------------------------
 // SearchV1Merge at level 0 for one query held by one wave.
+//
+// The reference's sorted array (SortArrBI, capacity max(ef, k)) is held in two
+// parts whose interleaving is the array: A, the bulk, sorted in LDS, and S, up
+// to 64 recently inserted items in registers (lane t holds S[t] and sab, the
+// number of A items before it; S[t] precedes A[i] iff sab <= i).  Insertions
+// go into S, so an expansion moves no LDS items; S is merged into A in one pass
+// only when the next insertions would not fit, and at the end.  Positions,
+// the last item, truncation at the capacity and the first unexpanded item are
+// computed over the interleaving.
 template <int D, int FUNC, int ST>
 __device__ __forceinline__ void v1merge_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, uint8_t ep,
                               int qi, int k, int ef, int lane, int h, uint64_t ltm) {
@@ -981,22 +995,89 @@ __device__ __forceinline__ void v1merge_query(const SearchArgs& a, const WaveLds
   float curd = dist_node<D, FUNC, ST>(w.qs, ix, cur, h);
   greedy_descent<D, FUNC, ST>(ix, w.qs, lane, h, cur, curd, ndu, nl);
 
-  int n = 1;
   if (lane == 0) {
     A[0] = make_uint2(__float_as_uint(curd), (uint32_t)cur);
     vis[cur] = ep;
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  int ce = 0;
+  int na = 1, ns = 0;  // items in A and in S
+  float s_key = 0.f;   // lane t < ns: S[t]'s key,
+  uint32_t s_id = 0;   //   internal id | kUsedBit once expanded,
+  int s_ab = 0;        //   and the number of A items before it
+  int ceA = 0;         // every A item below ceA is expanded
   int pf_node = -1, pf_t = -1;  // adjacency row of the node predicted to expand next
-  while (ce < min(n, ef)) {
+  int last_node = -1;
+  // per-phase device time (100 MHz ticks) when counters are collected: pick +
+  // adjacency, visited, distances, admission + prediction, sort, rank search,
+  // insertion + merge of S into A, advance
+  const bool timed = a.stats != nullptr;
+  unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint64_t tmark = timed ? __builtin_amdgcn_s_memrealtime() : 0;
+  auto lap = [&](int p) __attribute__((always_inline)) {
+    if (timed) {
+      const uint64_t now = __builtin_amdgcn_s_memrealtime();
+      ph[p] += now - tmark;
+      tmark = now;
+    }
+  };
+  auto rl_f = [](float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); };
+  auto rl_i = [](int v, int l) { return __builtin_amdgcn_readlane(v, l); };
+  // merges S into A: one pass moving A items up (place_items); the cursor
+  // moves to the new index of the first item that may be unexpanded
+  auto flush = [&]() __attribute__((always_inline)) {
+    if (ns == 0) return;
+    const int c0 = min(ceA, na);
+    int nc = c0 + __popcll(__ballot(lane < ns && s_ab <= c0));
+    const uint64_t su = __ballot(lane < ns && !(s_id & kUsedBit));
+    if (su) {
+      const int t = __ffsll((long long)su) - 1;
+      nc = min(nc, rl_i(s_ab, t) + t);
+    }
     WSYNC();
-    const uint2 e = A[ce];
-    const int node = (int)(e.y & ~kUsedBit);
-    if (lane == 0) A[ce].y = e.y | kUsedBit;
-    ++ce;
+    place_items(A, na, C, lane, ns, s_ab, make_uint2(__float_as_uint(s_key), s_id));
+    na += ns;
+    ns = 0;
+    ceA = nc;
+    WSYNC();
+  };
+
+  for (;;) {
+    WSYNC();
+    // the first unexpanded item (the reference's currElem after its advance,
+    // hnsw_distfunc_opt.cc:272-273): the first one of A at or after ceA or the
+    // first one of S, whichever comes first in the array
+    ceA = first_unused(A, ceA, na, lane);
+    const uint64_t su = __ballot(lane < ns && !(s_id & kUsedBit));
+    const int ts = su ? __ffsll((long long)su) - 1 : ns;
+    const int abt = rl_i(s_ab, min(ts, 63));
+    const bool from_a = ceA < na && (ts == ns || ceA < abt);
+    const int n = na + ns;
+    const int vpos = from_a ? ceA + __popcll(__ballot(lane < ns && s_ab <= ceA)) : (ts < ns ? abt + ts : n);
+    if (vpos >= min(n, ef)) break;
+    if (nexp >= (unsigned long long)ix.n) {  // every expansion takes a new node
+      if (a.err) atomicOr(a.err, 2);
+      break;
+    }
+    int node;
+    if (from_a) {
+      const uint2 e = A[ceA];
+      node = (int)(e.y & ~kUsedBit);
+      if (lane == 0) A[ceA].y = e.y | kUsedBit;
+      ++ceA;
+    } else {
+      node = (int)((uint32_t)rl_i((int)s_id, ts) & ~kUsedBit);
+      if (lane == ts) s_id |= kUsedBit;
+    }
+    node = checked_node(node, (int)ix.n, a.err);
+    if (node == last_node) {  // an item expanded twice: the state is corrupted
+      if (a.err) atomicOr(a.err, 2);
+      break;
+    }
+    last_node = node;
     ++nexp;
-    const float topk = akey(A, n - 1);
+    // the last item's key (top_key)
+    const bool last_s = ns > 0 && rl_i(s_ab, max(ns - 1, 0)) == na;
+    const float topk = last_s ? rl_f(s_key, max(ns - 1, 0)) : (na > 0 ? akey(A, na - 1) : kInf());
     const int nsnap = n;
 
     // adjacency list (one 128-byte row for maxM0 <= 32), usually prefetched
@@ -1009,6 +1090,7 @@ __device__ __forceinline__ void v1merge_query(const SearchArgs& a, const WaveLds
     }
     const bool valid = t >= 0;
     nl += 1 + __popcll(__ballot(valid));
+    lap(0);
     bool fresh = false;
     if (valid) fresh = vis[t] != ep;
     if (ix.has_dups) {
@@ -1024,27 +1106,34 @@ __device__ __forceinline__ void v1merge_query(const SearchArgs& a, const WaveLds
     if (fresh) w.cand[__popcll(fm & ltm)] = t;
     nd0 += ncand;
     WSYNC();
-    for (int b = 0; b < ncand; b += 32) {
-      const int s = b + (lane >> 1);
-      const bool ok = s < ncand;
-      const int id = ok ? w.cand[s] : node;
-      const float dd = dist_node<D, FUNC, ST>(w.qs, ix, id, h);
-      if (ok && h == 0) w.cdist[s] = dd;
-    }
+    lap(1);
+    dist_cands<D, FUNC, ST, 3>(w.qs, ix, w.cand, w.cdist, ncand, lane, h, node, a.err);
     WSYNC();
+    lap(2);
     const float dj = lane < ncand ? w.cdist[lane] : 0.f;
     const int cj = lane < ncand ? w.cand[lane] : 0;
     const bool keep = lane < ncand && (dj < topk || nsnap < ef);
     const uint64_t km = __ballot(keep);
     const int m = __popcll(km);
 
-    // The node expanded next is the first unused item at or after ce, unless
-    // a new item sorts before it: start loading its adjacency row now so the
-    // load overlaps the sort and merge below.
+    // The node expanded next is the first unexpanded item, unless a new item
+    // sorts before it: start loading its adjacency row now so the load
+    // overlaps the sort and insertion below.
     {
-      const int p = first_unused(A, ce, n, lane);
-      float nk = p < n ? akey(A, p) : kInf();
-      int nid = p < n ? (int)(A[p].y & ~kUsedBit) : -1;
+      ceA = first_unused(A, ceA, na, lane);  // A changes only in flush, which resets ceA
+      const uint64_t su2 = __ballot(lane < ns && !(s_id & kUsedBit));
+      const int ts2 = su2 ? __ffsll((long long)su2) - 1 : ns;
+      const bool a2 = ceA < na && (ts2 == ns || ceA < rl_i(s_ab, min(ts2, 63)));
+      float nk = kInf();
+      int nid = -1;
+      if (a2) {
+        const uint2 e2 = A[ceA];
+        nk = __uint_as_float(e2.x);
+        nid = (int)(e2.y & ~kUsedBit);
+      } else if (ts2 < ns) {
+        nk = rl_f(s_key, ts2);
+        nid = (int)((uint32_t)rl_i((int)s_id, ts2) & ~kUsedBit);
+      }
       float bk = keep ? dj : kInf();
       int bi = keep ? cj : -1;
 #pragma unroll
@@ -1057,13 +1146,15 @@ __device__ __forceinline__ void v1merge_query(const SearchArgs& a, const WaveLds
         }
       }
       const int guess = (bi >= 0 && bk < nk) ? bi : nid;
-      if (guess >= 0 && (p < min(n, ef) || bk < nk)) {
+      if (guess >= 0) {
         pf_node = guess;
-        pf_t = lane < ix.maxM0 ? ix.l0[(int64_t)guess * ix.l0stride + lane] : -1;
+        pf_t = lane < ix.maxM0 ? ix.l0[(int64_t)checked_node(guess, (int)ix.n, a.err) * ix.l0stride + lane]
+                               : -1;
       } else {
         pf_node = -1;
       }
     }
+    lap(3);
 
     if (m > 0) {
       if (keep) {
@@ -1072,14 +1163,15 @@ __device__ __forceinline__ void v1merge_query(const SearchArgs& a, const WaveLds
         w.bid[r] = cj;
       }
       WSYNC();
-      // stable ranks; an equal pair among more than 16 items needs the
-      // exact std::sort order (insertion sort is stable up to 16 items)
+      // the reference sorts the admitted items with std::sort: stable ranks;
+      // an equal pair among more than 16 items needs the exact std::sort
+      // order (insertion sort is stable up to 16 items)
       const float mk = lane < m ? w.bkey[lane] : 0.f;
       const int mid = lane < m ? w.bid[lane] : 0;
       int rank = 0;
       bool tie = false;
       for (int j = 0; j < m; ++j) {  // uniform loop: keys broadcast by readlane
-        const float kj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mk), j));
+        const float kj = rl_f(mk, j);
         rank += (kj < mk) || (kj == mk && j < lane);
         tie |= (kj == mk) && (j != lane);
       }
@@ -1099,47 +1191,82 @@ __device__ __forceinline__ void v1merge_query(const SearchArgs& a, const WaveLds
       WSYNC();
       const float bj = lane < m ? w.skey[lane] : 0.f;
       const int bidj = lane < m ? w.sid[lane] : 0;
-      const int Lj = lane < m ? lower_count(A, n, bj) : 0;
-      const int Uj = lane < m ? upper_count(A, n, bj) : 0;
-      const bool slow = anytie || (__ballot(lane < m && Lj != Uj) != 0);
+      lap(4);
+      if (ns + m > 64) flush();
+      lap(6);
+      // items of A below bj (Lj) and not above it (Uj); the same over S
+      const int Lj = lane < m ? lower_count(A, na, bj) : 0;
+      int Uj = Lj;
+      if (lane < m)
+        while (Uj < na && akey(A, Uj) == bj) ++Uj;
+      int LSj = 0, USj = 0, s_up = 0;  // s_up: new items below S[lane]
+      for (int j = 0; j < m; ++j) {
+        const float b = rl_f(bj, j);
+        const int ls = __popcll(__ballot(lane < ns && s_key < b));
+        const int us = __popcll(__ballot(lane < ns && s_key <= b));
+        if (lane == j) {
+          LSj = ls;
+          USj = us;
+        }
+        s_up += b < s_key ? 1 : 0;
+      }
+      const bool slow = anytie || (__ballot(lane < m && (Lj != Uj || LSj != USj)) != 0);
+      lap(5);
       if (!slow) {
-        // distinct keys: sequential insertion == merge truncated to C
-        const int L0 = __shfl(Lj, 0);
-        place_items(A, n, C, lane, m, Lj, make_uint2(__float_as_uint(bj), (uint32_t)bidj));
-        n = min(C, n + m);
-        if (L0 < ce) ce = L0;
+        // distinct keys: sequential insertion == merge truncated to the
+        // capacity.  New item j takes S position LSj + j with Lj A items
+        // before it; S[t] moves up by the new items below it.
+        float* xk = w.skey;
+        int* xi = w.sid;
+        int* xa = w.bid;
+        WSYNC();
+        if (lane < ns) {
+          const int d = lane + s_up;
+          xk[d] = s_key;
+          xi[d] = (int)s_id;
+          xa[d] = s_ab;
+        }
+        if (lane < m) {
+          const int d = LSj + lane;
+          xk[d] = bj;
+          xi[d] = bidj;
+          xa[d] = Lj;
+        }
+        WSYNC();
+        ns += m;
+        if (lane < ns) {
+          s_key = xk[lane];
+          s_id = (uint32_t)xi[lane];
+          s_ab = xa[lane];
+        }
+        // the last items fall off
+        while (na + ns > C) {
+          if (ns > 0 && rl_i(s_ab, ns - 1) == na) --ns; else --na;
+        }
       } else {
-        // Equal keys: replay the reference's item-by-item insertion.  The
-        // items already inserted (S, at most 64) stay in registers, lane t
-        // holding S[t] with the count of surviving A items before it; the
-        // live array is the interleaving of A[0..na) and S, so each step
-        // only needs counts over A (precomputed) and ballots over S, and A is
-        // moved once at the end.
+        // Equal keys: replay the reference's item-by-item insertion
+        // (push_or_replace_non_empty_exp, sort_arr_bi.h:159-199) over the
+        // interleaving: counts over A are precomputed, counts over S are
+        // ballots, and S takes the items.
         ++nties;
-        float s_key = 0.f;
-        int s_id = 0, s_ab = 0;
-        int na = n, ns = 0;
         for (int j = 0; j < m; ++j) {
-          const float b = __shfl(bj, j);
-          const int bi = __shfl(bidj, j);
-          const int LA = min(__shfl(Lj, j), na);
-          const int UA = min(__shfl(Uj, j), na);
+          const float b = rl_f(bj, j);
+          const int bi = rl_i(bidj, j);
+          const int LA = min(rl_i(Lj, j), na);
+          const int UA = min(rl_i(Uj, j), na);
           const int nv = na + ns;
-          const int ab_last = __shfl(s_ab, ns > 0 ? ns - 1 : 0);
-          const bool last_is_s = ns > 0 && ab_last == na;
-          const float lastkey = last_is_s ? __shfl(s_key, ns - 1) : akey(A, na - 1);
-          int ret;
+          const bool last_is_s = ns > 0 && rl_i(s_ab, max(ns - 1, 0)) == na;
+          const float lastkey = last_is_s ? rl_f(s_key, max(ns - 1, 0)) : (na > 0 ? akey(A, na - 1) : kInf());
           if (lastkey <= b) {
             // append after an equal last key while there is room
             if (nv < C) {
               if (lane == ns) {
                 s_key = b;
                 s_ab = na;
-                s_id = bi;
+                s_id = (uint32_t)bi;
               }
               ++ns;
             }
-            ret = nv;
           } else {
             const int L = LA + __popcll(__ballot(lane < ns && s_key < b));
             const int U = UA + __popcll(__ballot(lane < ns && s_key <= b));
@@ -1159,7 +1286,7 @@ __device__ __forceinline__ void v1merge_query(const SearchArgs& a, const WaveLds
             const int t0 = __popcll(__ballot(lane < ns && s_ab + lane < pos));
             const float uk = __shfl_up(s_key, 1);
             const int ua = __shfl_up(s_ab, 1);
-            const int ui = __shfl_up(s_id, 1);
+            const uint32_t ui = (uint32_t)__shfl_up((int)s_id, 1);
             if (lane > t0 && lane <= ns) {
               s_key = uk;
               s_ab = ua;
@@ -1168,28 +1295,24 @@ __device__ __forceinline__ void v1merge_query(const SearchArgs& a, const WaveLds
             if (lane == t0) {
               s_key = b;
               s_ab = pos - t0;
-              s_id = bi;
+              s_id = (uint32_t)bi;
             }
             ++ns;
-            ret = pos;
           }
-          if (ret < ce) ce = ret;
         }
-        // materialise: A item i moves up by the number of S items before it
-        place_items(A, na, C, lane, ns, s_ab, make_uint2(__float_as_uint(s_key), (uint32_t)s_id));
-        n = na + ns;
       }
+      lap(6);
     }
-    WSYNC();
-    // the next unexpanded item (hnsw_distfunc_opt.cc:272-273)
-    ce = first_unused(A, ce, n, lane);
     // the marks of this expansion must land before the next one reads them
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    lap(7);
   }
-  WSYNC();
+  flush();
+  if (timed && lane == 0)
+    for (int p = 0; p < 8; ++p) atomicAdd(&a.stats->phase_ticks[p], ph[p]);
 
   const QueryCounters c{nexp, nd0, ndu, nl, nties};
-  finish_query<D, FUNC, ST>(a, w, n, qi, k, lane, h, c);
+  finish_query<D, FUNC, ST>(a, w, na, qi, k, lane, h, c);
 }
 
 
@@ -2064,13 +2187,7 @@ __device__ __forceinline__ void old_query(const SearchArgs& a, const WaveLds& w,
     if (fresh) w.cand[__popcll(fm & ltm)] = t;
     nd0 += ncand;
     WSYNC();
-    for (int b = 0; b < ncand; b += 32) {
-      const int s = b + (lane >> 1);
-      const bool ok = s < ncand;
-      const int id = ok ? w.cand[s] : node;
-      const float dd = dist_node<D, FUNC, ST>(w.qs, ix, id, h);
-      if (ok && h == 0) w.cdist[s] = dd;
-    }
+    dist_cands<D, FUNC, ST, 3>(w.qs, ix, w.cand, w.cdist, ncand, lane, h, node, a.err);
     WSYNC();
     if (lane == 0) {
       for (int j = 0; j < ncand; ++j) {

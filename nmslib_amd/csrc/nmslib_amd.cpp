@@ -791,7 +791,7 @@ void finish_stats(nmsamd_index* h) {
   int err = 0;
   hip_check(hipMemcpy(&err, h->last_err, sizeof(int), hipMemcpyDeviceToHost), "status");
   if (err & 2)
-    fail(NMSAMD_ERUNTIME, "internal error: the search state held a node id outside the index");
+    fail(NMSAMD_ERUNTIME, "internal error: corrupted search state (a node id outside the index, or an item expanded twice)");
   if (err)
     fail(NMSAMD_ERUNTIME, "SearchOld candidate heap overflow (more than 64*ef admitted items)");
   float ms = 0;
@@ -1350,9 +1350,11 @@ int nmsamd_last_launch(const nmsamd_index* h, int64_t* out5) {
   });
 }
 
-int nmsamd_last_phase_us(const nmsamd_index* h, double* out8) {
+int nmsamd_last_phase_us(const nmsamd_index* hc, double* out8) {
   return guarded([&] {
-    if (!h || !out8) fail(NMSAMD_EINVAL, "null argument");
+    if (!hc || !out8) fail(NMSAMD_EINVAL, "null argument");
+    auto* h = const_cast<nmsamd_index*>(hc);  // device entry point: read back now (see last_stats)
+    if (h->ev1 && hipEventQuery(h->ev1) == hipSuccess) finish_stats(h);
     for (int p = 0; p < 8; ++p) out8[p] = h->last_stats.phase_ticks[p] * 0.01;  // 100 MHz
   });
 }
@@ -1611,7 +1613,7 @@ int nmsamd_server_fetch(nmsamd_server* s, int lane, int32_t* ids, float* dists, 
     hip_check(hipEventSynchronize(L.done), "server lane");
     L.busy = false;
     if (*L.herr & 2)
-      fail(NMSAMD_ERUNTIME, "internal error: the search state held a node id outside the index");
+      fail(NMSAMD_ERUNTIME, "internal error: corrupted search state (a node id outside the index, or an item expanded twice)");
     const size_t out = (size_t)L.nq * s->max_k;
     std::memcpy(ids, L.hids, sizeof(int32_t) * out);
     std::memcpy(dists, L.hdists, sizeof(float) * out);
