@@ -105,6 +105,9 @@ struct SearchArgs {
   int64_t old_heap_cap;
   int* err;                  // bit 1: a SearchOld candidate heap overflowed; bit 2: the
                              // search state held a node id outside the index or expanded an item twice
+  int defer;                 // two-wave beam queries: the first wave rebuilds the sorted
+                             // array while the second computes the next step's distances
+                             // (needs the second items buffer after the command words)
 };
 
 // algoType (hnsw.cc:491-501); hybrid picks SearchOld for ef >= 1000 per query

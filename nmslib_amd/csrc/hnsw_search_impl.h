@@ -709,6 +709,8 @@ struct WaveLds {
   int* ctl;        // [24] two-wave queries: two alternating 8-word commands
                    // from the first wave to the second, and the second
                    // wave's count of admitted items (word 16)
+  uint2* items2;   // [ent] two-wave queries with a deferred rebuild: the items
+                   // buffer of every other step (the launch allocates it)
 };
 
 __device__ __forceinline__ WaveLds carve(unsigned char* base, int cap, int dstride, int ent) {
@@ -731,6 +733,7 @@ __device__ __forceinline__ WaveLds carve(unsigned char* base, int cap, int dstri
   w.pfq = w.pfa + 24;
   w.pfpos = reinterpret_cast<int*>(w.pfq + 16);
   w.ctl = w.pfpos + 24;
+  w.items2 = reinterpret_cast<uint2*>(w.ctl + 24);
   return w;
 }
 
@@ -1445,11 +1448,11 @@ struct PairLink {
   const uint2* items;  // base of the admitted-items buffer
   int* lbuf;           // the second wave's ranks of the new items (kCmdRank)
   __device__ __forceinline__ void post(int lane, int cmd, int p1 = 0, int p2 = 0, int p3 = 0,
-                                       int p4 = 0, int p5 = 0, int p6 = 0) {
+                                       int p4 = 0, int p5 = 0, int p6 = 0, int p7 = 0) {
     int4* c = reinterpret_cast<int4*>(ctl + 8 * (seq & 1));
     if (lane == 0) {
       c[0] = make_int4(cmd, p1, p2, p3);
-      c[1] = make_int4(p4, p5, p6, 0);
+      c[1] = make_int4(p4, p5, p6, p7);
     }
     ++seq;
     pair_sync();
@@ -1489,6 +1492,55 @@ __device__ __forceinline__ void walk_pair(uint2* A, const uint2* X, const uint4*
     for (int u = 0; u < 8; ++u)
       if (pos[u] >= 0) A[pos[u]] = val[u];
   }
+}
+
+// The rebuild walk of sort_merge by one wave: output chunks chi down to lo
+// (>= clo, the chunk of meta[0]), eight in flight.  Output chunk c reads only
+// below 64 (c + 1) and writes [64 c, 64 (c + 1)), so going top-down the reads
+// of lower chunks may run ahead of the writes, and the walk may stop at any
+// chunk and resume below it later.
+__device__ __forceinline__ void walk_all(uint2* A, const uint2* X, const uint4* meta, int clo,
+                                         int chi, int nout, int lane, int lo = -1) {
+  if (lo < clo) lo = clo;
+  for (int c = chi; c >= lo; c -= 8) {
+    uint4 md[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) md[u] = meta[max(c - u, lo) - clo];
+    uint2 v[8];
+    int pos[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int cc = c - u;
+      const int p = (cc << 6) + lane;
+      // marks below this lane: mbcnt counts the set bits of the mask below it
+      const int bl = (int)md[u].z + (int)__builtin_amdgcn_mbcnt_hi(
+                                        md[u].y, __builtin_amdgcn_mbcnt_lo(md[u].x, 0u));
+      const bool mine = ((lane < 32 ? md[u].x : md[u].y) >> (lane & 31)) & 1u;
+      const bool ok = cc >= lo && p < nout;
+      const uint2* src = mine ? X + bl : A + (p - bl);
+      v[u] = *(ok ? src : A);
+      pos[u] = ok ? p : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (pos[u] >= 0) A[pos[u]] = v[u];
+  }
+}
+
+// Item p of the array a merge makes while its rebuild walk has covered only
+// the chunks above chi: the merge records of chunks clo..chi (see sort_merge)
+// say whether position p takes a new item of X or an item of A moved up;
+// positions below chunk clo keep A's, those above chi are rebuilt.  p is the
+// same on every lane.
+__device__ __forceinline__ uint2 merged_item(const uint2* A, const uint2* X, const uint4* meta,
+                                             int clo, int chi, int p) {
+  const int c = p >> 6;
+  if (c < clo || c > chi) return A[p];
+  const uint4 md = meta[c - clo];
+  const int b = p & 63;
+  const uint64_t mask = ((uint64_t)md.y << 32) | md.x;
+  const int bl = (int)md.z + __popcll(mask & ((1ull << b) - 1ull));
+  return ((mask >> b) & 1ull) ? X[bl] : A[p - bl];
 }
 
 // Ranks of the sorted new items X[0..m) (item r * 64 + lane in k[r], id[r])
@@ -1569,11 +1621,15 @@ __device__ __forceinline__ int rank_and_mark(const uint2* A, int n, int m, int n
 //
 // before_walk() runs once the records are built, before the rebuild (or in
 // place of it when nothing lands).
+//
+// Two waves (PAIR) with `defer`: the rebuild is left to the caller, who gets
+// (clo, chi, nout) of the walk in `pend`, or clo = -1 when nothing lands below
+// nout.
 template <int R, bool PAIR, class F, class G, class H>
 __device__ __forceinline__ int sort_merge(uint2* A, int n_, int C_, uint2* X, int m_,
                                           uint4* meta, int lane, F&& after_sort, G&& lap,
                                           const int2& nxt, int* first, H&& before_walk,
-                                          PairLink& link) {
+                                          PairLink& link, bool defer, int3& pend) {
   const int n = __builtin_amdgcn_readfirstlane(n_);
   const int C = __builtin_amdgcn_readfirstlane(C_);
   const int m = __builtin_amdgcn_readfirstlane(m_);
@@ -1627,7 +1683,8 @@ __device__ __forceinline__ int sort_merge(uint2* A, int n_, int C_, uint2* X, in
     const int P0 = __builtin_amdgcn_readfirstlane(Lb[0]);
     before_walk();
     lap(5);
-    if (P0 >= nout) return P0;
+    if (defer) pend = make_int3(P0 >= nout ? -1 : P0 >> 6, (nout - 1) >> 6, nout);
+    if (P0 >= nout || defer) return P0;
     const int clo = P0 >> 6, chi = (nout - 1) >> 6;
     link.post(lane, kCmdWalk, clo, chi, nout, (int)(X - link.items));
     walk_pair(A, X, meta, clo, chi, nout, lane, 0);
@@ -1664,29 +1721,7 @@ __device__ __forceinline__ int sort_merge(uint2* A, int n_, int C_, uint2* X, in
   const int clo = P0 >> 6, chi = (nout - 1) >> 6;
   before_walk();
   lap(5);
-  for (int c = chi; c >= clo; c -= 8) {
-    uint4 md[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) md[u] = meta[max(c - u, clo) - clo];
-    uint2 v[8];
-    int pos[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int cc = c - u;
-      const int p = (cc << 6) + lane;
-      // marks below this lane: mbcnt counts the set bits of the mask below it
-      const int bl = (int)md[u].z + (int)__builtin_amdgcn_mbcnt_hi(
-                                        md[u].y, __builtin_amdgcn_mbcnt_lo(md[u].x, 0u));
-      const bool mine = ((lane < 32 ? md[u].x : md[u].y) >> (lane & 31)) & 1u;
-      const bool ok = cc >= clo && p < nout;
-      const uint2* src = mine ? X + bl : A + (p - bl);
-      v[u] = *(ok ? src : A);
-      pos[u] = ok ? p : -1;
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-      if (pos[u] >= 0) A[pos[u]] = v[u];
-  }
+  walk_all(A, X, meta, clo, chi, nout, lane);
   lap(6);
   return P0;
 }
@@ -1744,6 +1779,23 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
   // visited marks (vn) load right after it.  The next step then takes the
   // surviving prefix of pfq without scanning the array.
   const bool claim = C == ef;
+  // Two waves with a.defer: a merge's rebuild waits until the next step's
+  // distances are handed to the second wave, and this wave runs it while they
+  // are computed (the array is not read in between: the next step's picks
+  // were claimed from the sorted items and the old array, and its last item
+  // comes from the merge records).  The items of consecutive steps alternate
+  // between two buffers, since the pending rebuild reads the last step's.
+  const bool defer = PAIR && a.defer != 0;
+  uint2* itb = w.items;
+  int3 pend = make_int3(-1, 0, 0);  // pend.x >= 0: chunks pend.y down to pend.x of pX's walk are due
+  const uint2* pX = w.items;
+  auto flush = [&]() __attribute__((always_inline)) {
+    if (pend.x >= 0) {
+      walk_all(A, pX, w.meta, pend.x, pend.y, pend.z, lane);
+      pend.x = -1;
+      WSYNC();
+    }
+  };
   int pf_n = 0;  // > 0: pfq[0..pf_n) claimed for the next step
   int2 nxt = make_int2(-1, 0);  // first unclaimed items (see sort_merge)
   int first = 0;
@@ -1828,14 +1880,15 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
     WSYNC();
     int nsel = 0;
     const bool claimed = pf_n > 0;
+    const uint2 last = pend.x >= 0 ? merged_item(A, pX, w.meta, pend.x, pend.y, n - 1) : A[n - 1];
     if (claimed) {
       // the claimed picks that survived the last merge (a prefix of pfq)
-      const uint2 last = A[n - 1];
       const uint2 q = lane < pf_n ? w.pfq[lane] : make_uint2(0u, 0u);
       const bool surv = lane < pf_n && !kid_less(__uint_as_float(last.x), last.y & ~kUsedBit,
                                                  __uint_as_float(q.x), q.y);
       nsel = __builtin_amdgcn_readfirstlane(__popcll(__ballot(surv)));
     } else {
+      flush();
       // the first W unexpanded items below min(n, ef), in position order
       const int lim = min(n, ef);
       for (int p = ce; p < lim && nsel < W; p += 64) {
@@ -1854,7 +1907,7 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
     }
     if (nsel == 0) break;
     nexp += nsel;
-    const float topk = akey(A, n - 1);
+    const float topk = __uint_as_float(last.x);
     const int nsnap = n;
     // clear the dedupe table (2 E slots) while the picks land
     for (int i = lane; i < E; i += 64) reinterpret_cast<uint2*>(w.htab)[i] = make_uint2(~0u, ~0u);
@@ -1953,14 +2006,24 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
       constexpr int P0 = ST == kF32 ? 0 : DistRound<D, FUNC, ST, DPATH>::value;
       constexpr int P1 = DistRound<D, FUNC, ST, 1>::value;
       const int r = (ncand + P0 + P1 - 1) / (P0 + P1);
-      h0 = max(0, ncand - P1 * r);
-      if (h0 < ncand) link.post(lane, kCmdDist, ncand, h0, __float_as_int(topk), nsnap, ef, cur);
+      // deferred rebuild: the second wave takes every candidate (this one
+      // rebuilds meanwhile; distances of its own, one vector per lane pair,
+      // measured slower)
+      h0 = defer ? 0 : max(0, ncand - P1 * r);
+      if (h0 < ncand)
+        link.post(lane, kCmdDist, ncand, h0, __float_as_int(topk), nsnap, ef, cur,
+                  (int)(itb - w.items));
+      if (defer && pend.x >= 0) {
+        lap(2);
+        flush();
+        lap(6);
+      }
     }
     dist_cands<D, FUNC, ST, DPATH>(w.qs, ix, w.cand, w.cdist, h0, lane, h, cur, a.err);
     WSYNC();
     lap(2);
     // keep what the snapshot admits; order does not matter (sorted next)
-    int m = admit(w, 0, h0, topk, nsnap, ef, lane, ltm, w.items);
+    int m = admit(w, 0, h0, topk, nsnap, ef, lane, ltm, itb);
     if constexpr (PAIR) {
       if (h0 < ncand) {
         link.post(lane, kCmdNop);  // the second wave's items are in
@@ -1970,8 +2033,8 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
         if (m < h0) {
           for (int b = 0; b < m1; b += 64) {
             const int i = b + lane;
-            const uint2 v = i < m1 ? w.items[h0 + i] : make_uint2(0u, 0u);
-            if (i < m1) w.items[m + i] = v;
+            const uint2 v = i < m1 ? itb[h0 + i] : make_uint2(0u, 0u);
+            if (i < m1) itb[m + i] = v;
           }
         }
         m += m1;
@@ -1985,44 +2048,49 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
       // pass only holds items the earlier ones do not); a single pass also
       // claims the next picks
       if (m <= 256 && claim) {
-        auto pred = [&]() __attribute__((always_inline)) { predict(w.items, m); };
-        int L0;
-        if (m <= 64) L0 = sort_merge<1, PAIR>(A, n, C, w.items, m, w.meta, lane, pred, lap, nxt, &first,
-                                     load_marks, link);
-        else if (m <= 128) L0 = sort_merge<2, PAIR>(A, n, C, w.items, m, w.meta, lane, pred, lap, nxt, &first,
-                                     load_marks, link);
-        else L0 = sort_merge<4, PAIR>(A, n, C, w.items, m, w.meta, lane, pred, lap, nxt, &first,
-                                     load_marks, link);
+        auto pred = [&]() __attribute__((always_inline)) { predict(itb, m); };
+        if (m <= 64) sort_merge<1, PAIR>(A, n, C, itb, m, w.meta, lane, pred, lap, nxt, &first,
+                                         load_marks, link, defer, pend);
+        else if (m <= 128) sort_merge<2, PAIR>(A, n, C, itb, m, w.meta, lane, pred, lap, nxt, &first,
+                                               load_marks, link, defer, pend);
+        else sort_merge<4, PAIR>(A, n, C, itb, m, w.meta, lane, pred, lap, nxt, &first,
+                                 load_marks, link, defer, pend);
         n = min(C, n + m);
         ce = first;
+        if (pend.x >= 0) {  // the rebuild reads these items: the next step takes the other buffer
+          pX = itb;
+          itb = itb == w.items ? w.items2 : w.items;
+        }
       } else {
         for (int b = 0; b < m; b += 256) {
           const int cnt = min(256, m - b);
           int L0;
-          if (cnt <= 64) L0 = sort_merge<1, PAIR>(A, n, C, w.items + b, cnt, w.meta, lane, no_predict, lap, nxt,
-                                          nullptr, no_predict, link);
-          else if (cnt <= 128) L0 = sort_merge<2, PAIR>(A, n, C, w.items + b, cnt, w.meta, lane, no_predict, lap, nxt,
-                                          nullptr, no_predict, link);
-          else L0 = sort_merge<4, PAIR>(A, n, C, w.items + b, cnt, w.meta, lane, no_predict, lap, nxt,
-                                          nullptr, no_predict, link);
+          if (cnt <= 64) L0 = sort_merge<1, PAIR>(A, n, C, itb + b, cnt, w.meta, lane, no_predict, lap, nxt,
+                                          nullptr, no_predict, link, false, pend);
+          else if (cnt <= 128) L0 = sort_merge<2, PAIR>(A, n, C, itb + b, cnt, w.meta, lane, no_predict, lap, nxt,
+                                          nullptr, no_predict, link, false, pend);
+          else L0 = sort_merge<4, PAIR>(A, n, C, itb + b, cnt, w.meta, lane, no_predict, lap, nxt,
+                                          nullptr, no_predict, link, false, pend);
           n = min(C, n + cnt);
           if (L0 < ce) ce = L0;
           WSYNC();
         }
       }
     } else if (claim) {
-      predict(w.items, 0);
+      predict(itb, 0);
       load_marks();
       ce = nxt.x >= 0 ? nxt.x : n;
       lap(4);
     }
     WSYNC();
     if (pf_n == 0) {
+      flush();
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       ce = first_unused(A, ce, n, lane);
     }
     lap(7);
   }
+  flush();
   WSYNC();
   if (timed && lane == 0)
     for (int p = 0; p < 8; ++p) atomicAdd(&a.stats->phase_ticks[p], ph[p]);
@@ -2057,9 +2125,9 @@ __device__ __forceinline__ void helper_rank(const WaveLds& w, const uint2* X, in
 
 // The second wave of a two-wave query (kModeBeamPair): carries out the
 // first wave's commands until it posts kCmdExit.
-//   kCmdDist (ncand, h0, topk, nsnap, ef, fallback id): distances of
-//     cand[h0..ncand) and the admitted ones as items at items[h0..); their
-//     number goes to ctl[16];
+//   kCmdDist (ncand, h0, topk, nsnap, ef, fallback id, items buffer offset):
+//     distances of cand[h0..ncand) and the admitted ones as items at
+//     items[offset + h0..); their number goes to ctl[16];
 //   kCmdRank (n, m, nout, offset of the new items in items): ranks and merge
 //     records of a merge pass (helper_rank);
 //   kCmdWalk (clo, chi, nout, offset of the new items in items): the lower
@@ -2082,10 +2150,11 @@ __device__ __forceinline__ void beam_helper(const SearchArgs& a, const WaveLds& 
       const int nsnap = __builtin_amdgcn_readfirstlane(c1.x);
       const int ef = __builtin_amdgcn_readfirstlane(c1.y);
       const int fb = __builtin_amdgcn_readfirstlane(c1.z);
+      uint2* itb = w.items + __builtin_amdgcn_readfirstlane(c1.w);
       dist_cands<D, FUNC, ST, 1>(w.qs, a.ix, w.cand + h0, w.cdist + h0, ncand - h0, lane, h, fb,
                                  a.err);
       WSYNC();
-      const int m1 = admit(w, h0, ncand, topk, nsnap, ef, lane, ltm, w.items + h0);
+      const int m1 = admit(w, h0, ncand, topk, nsnap, ef, lane, ltm, itb + h0);
       if (lane == 0) w.ctl[16] = m1;
     } else if (cmd == kCmdRank) {
       const int n = __builtin_amdgcn_readfirstlane(c0.y);

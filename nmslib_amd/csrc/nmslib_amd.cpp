@@ -579,6 +579,17 @@ int lds_wave_bytes(int cap, int dstride, int ent, bool beam) {
   return (b + 15) / 16 * 16;
 }
 
+// LDS a two-wave block reserves: more than a quarter of the CU's, so that a
+// fourth block never fits, plus the second items buffer of the deferred
+// rebuild (8 * ent bytes) where that keeps the CU's number of queries.
+int pair_lds(int lds, int ent, bool* defer) {
+  const int floor = (int)(kLdsPerCu / 4) + 16;
+  const int plain = std::max(lds, floor);
+  const int with2 = std::max(lds + 8 * ent, floor);
+  *defer = kLdsPerCu / (size_t)with2 == kLdsPerCu / (size_t)plain;
+  return *defer ? with2 : plain;
+}
+
 // The handle's launches share its work counter, error word, visited slots,
 // epochs and events: a launch on another stream than the last one starts only
 // after the last has finished (its end is ev1; the device entry point returns
@@ -678,7 +689,8 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   // more than a quarter of the CU's LDS, so a fourth block never fits.
   const size_t fit = kLdsPerCu / lds;
   const bool pair = beam && (h->beam_waves == 2 || (h->beam_waves == 0 && fit <= 3));
-  const int lds_launch = pair ? std::max(lds, (int)(kLdsPerCu / 4) + 16) : lds;
+  bool defer = false;
+  const int lds_launch = pair ? pair_lds(lds, ent, &defer) : lds;
   const int wpb = pair ? 1 : (int)std::max<size_t>(1, std::min<size_t>(kWavesPerBlock, fit));
   const int per_cu = (int)std::max<size_t>(
       1, std::min<size_t>(pair ? 3 : 8, kLdsPerCu / ((size_t)lds_launch * wpb)));
@@ -772,6 +784,7 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   a.old_heap_key = old_key;
   a.old_heap_id = old_id;
   a.old_heap_cap = heap_cap;
+  a.defer = defer ? 1 : 0;
   hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
   hip_check(launch_search(a, blocks, wpb, lds_launch, mode, s), "search kernel launch");
   hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
@@ -1410,6 +1423,7 @@ struct nmsamd_server {
   // index's setters change afterwards
   int storage = kF32, rerank = 0, beam = 0;
   int pair = 0;  // two waves per query (kModeBeamPair), from the LDS footprint
+  int defer = 0;  // ... with the deferred rebuild (a second items buffer)
   DevIndex ix{};
   int slots_per_lane = 0;
   bool use_graph = false;
@@ -1452,6 +1466,7 @@ SearchArgs server_args(nmsamd_server* s, ServerLane& L) {
   a.ent = s->ent;
   a.algo = kAlgoV1Merge;
   a.slot_base = L.slot_base;
+  a.defer = s->defer;
   return a;
 }
 
@@ -1503,7 +1518,11 @@ int nmsamd_server_open(nmsamd_index* h, int lanes, int max_queries, int max_ef, 
     // holds a CU to four waves
     const size_t fit = kLdsPerCu / s->lds;
     s->pair = (h->beam > 0 && h->beam_waves != 1 && (h->beam_waves == 2 || fit <= 3)) ? 1 : 0;
-    if (s->pair) s->lds = std::max(s->lds, (int)(kLdsPerCu / 4) + 16);
+    if (s->pair) {
+      bool defer = false;
+      s->lds = pair_lds(s->lds, s->ent, &defer);
+      s->defer = defer ? 1 : 0;
+    }
     s->wpb = s->pair ? 1 : (int)std::max<size_t>(1, std::min<size_t>(kWavesPerBlock, fit));
     const int per_cu = (int)std::max<size_t>(
         1, std::min<size_t>(s->pair ? 3 : 8, kLdsPerCu / ((size_t)s->lds * s->wpb)));
