@@ -427,6 +427,8 @@ def main():
                     help="build the index with this library's createIndex on N host threads "
                          "(M=16, efConstruction=500, as the reference-built pack) instead of "
                          "loading the reference's file; 0 = load the reference's file")
+    ap.add_argument("--visited-marks", type=int, default=0,
+                    help="visited marks: 0 = by size, 1 = epoch bytes, 2 = bits (setVisitedMarks)")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="only a 1 GiB copy and one search launch (for rocprofv3 --pmc runs)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "hnsw_search_kernel_halfnormal1m_traffic.md"),
@@ -472,6 +474,7 @@ def main():
         idx.loadIndex(path)
         log(f"[bench] loadIndex {time.time() - t:.1f}s {idx.info()}")
     dev = torch.device("cuda:0")
+    idx.setVisitedMarks(args.visited_marks)
 
     if args.pmc_pass:
         # short fixed pass for hardware-counter runs (rocprofv3 --pmc): a plain

@@ -172,10 +172,10 @@ int nmsamd_set_precision(nmsamd_index* h, int storage, int rerank_mult);
 int nmsamd_set_beam(nmsamd_index* h, int beam);
 
 /* Waves per query in beam mode: 1, 2 (a block of two waves per query: the
- * second computes part of each step's distances and rebuilds half of the
- * sorted array; at most three queries per CU), or 0 (default) = two where the
- * LDS footprint limits a CU to three queries.  Results do not depend on it.  No
- * reference counterpart. */
+ * second computes each step's distances while the first rebuilds the sorted
+ * array from the last merge; at most three queries per CU), or 0 (default) =
+ * two where the LDS footprint limits a CU to three queries.  Results do not
+ * depend on it.  No reference counterpart. */
 int nmsamd_set_beam_waves(nmsamd_index* h, int waves);
 
 /* Vector loads in flight in the one-wave beam kernels: 1 = the plain kernel
@@ -185,6 +185,15 @@ int nmsamd_set_beam_waves(nmsamd_index* h, int waves);
  * where the LDS footprint already holds a CU to four waves.  Results do not
  * depend on it.  No reference counterpart (an autotuner knob). */
 int nmsamd_set_beam_gather(nmsamd_index* h, int gather);
+
+/* Visited marks of the batch entry points' searches (the reference keeps a
+ * byte per node with a version counter, hnsw.h:568-591): 1 = epoch bytes (n
+ * bytes per resident query, cleared when the epoch wraps), 2 = bits (n / 8
+ * bytes, cleared when each query starts), 0 (default) = bits where clearing
+ * n / 8 bytes is small next to about efSearch * maxM0 mark probes per query.
+ * Results do not depend on it.  Servers choose at open by their max_ef.  No
+ * reference counterpart (a tuning knob). */
+int nmsamd_set_visited_marks(nmsamd_index* h, int marks);
 
 /* Per-call counters of the last search (expansions, level-0 distance
  * evaluations, upper-level distance evaluations, adjacency ints read,

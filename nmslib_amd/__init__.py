@@ -75,6 +75,7 @@ SIGNATURES = {
     "nmsamd_last_query_us": (C.c_int, [_P, _P, _I64]),
     "nmsamd_set_beam": (C.c_int, [_P, C.c_int]),
     "nmsamd_set_beam_gather": (C.c_int, [_P, C.c_int]),
+    "nmsamd_set_visited_marks": (C.c_int, [_P, C.c_int]),
     "nmsamd_set_beam_waves": (C.c_int, [_P, C.c_int]),
     "nmsamd_brute_force": (C.c_int, [_P, _P, _I64, _I64, _I32, _P, _P, _P]),
     "nmsamd_last_phase_us": (C.c_int, [_P, _P]),
@@ -246,9 +247,10 @@ class FloatIndex:
         _check(load_library().nmsamd_set_beam(self._h, int(beam)))
 
     def setBeamWaves(self, waves: int = 0):
-        """Waves per beam query: 1, 2 (the second wave computes part of each
-        step's distances and rebuilds half of the sorted array) or 0 = chosen
-        from the LDS footprint.  Results do not depend on it."""
+        """Waves per beam query: 1, 2 (the second wave computes each step's
+        distances while the first rebuilds the sorted array from the last
+        merge) or 0 = chosen from the LDS footprint.  Results do not depend
+        on it."""
         _check(load_library().nmsamd_set_beam_waves(self._h, int(waves)))
 
     def setBeamGather(self, gather: int = 0):
@@ -256,6 +258,12 @@ class FloatIndex:
         waves per SIMD), 2 = wide (one wave per SIMD, every load of a round in
         flight), 0 = chosen from the LDS footprint.  Results do not depend on it."""
         _check(load_library().nmsamd_set_beam_gather(self._h, int(gather)))
+
+    def setVisitedMarks(self, marks: int = 0):
+        """Visited marks: 1 = epoch bytes, 2 = bits cleared per query, 0 =
+        bits where clearing n / 8 bytes is small next to efSearch * maxM0
+        probes.  Results do not depend on it."""
+        _check(load_library().nmsamd_set_visited_marks(self._h, int(marks)))
 
     def collectStats(self, on: bool = True):
         _check(load_library().nmsamd_collect_stats(self._h, int(bool(on))))

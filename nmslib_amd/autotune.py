@@ -5,8 +5,8 @@ except storage and rerank, whose recall is measured):
 
 * beam width W: expansions per step, 0 = the reference's exact V1Merge,
   1..16 = beam mode;
-* waves per query in beam mode: 1, or 2 (the second wave computes part of
-  each step's distances and rebuilds half of the sorted array);
+* waves per query in beam mode: 1, or 2 (the second wave computes each
+  step's distances while the first rebuilds the sorted array);
 * gather width of the one-wave kernels: plain (two waves per SIMD, one FP32 /
   two BF16 / four FP8 vectors per lane pair in flight) or wide (one wave per
   SIMD, two / five / eight);

@@ -108,6 +108,7 @@ struct SearchArgs {
   int defer;                 // two-wave beam queries: the first wave rebuilds the sorted
                              // array while the second computes the next step's distances
                              // (needs the second items buffer after the command words)
+  int vbits;                 // visited marks as bits (cleared per query), else epoch bytes
 };
 
 // algoType (hnsw.cc:491-501); hybrid picks SearchOld for ef >= 1000 per query
@@ -115,8 +116,9 @@ enum AlgoMode : int { kAlgoHybrid = 0, kAlgoOld = 1, kAlgoV1Merge = 2 };
 
 // Kernel variants: every search mode, or beam mode only (lower register
 // pressure), the latter also with all of a step's vector loads in flight, or
-// with two waves per query (a block of two waves: the second computes part of
-// each step's distances and rebuilds half of the sorted array).
+// with two waves per query (a block of two waves: the second computes each
+// step's distances, and the first rebuilds the sorted array meanwhile or, where
+// LDS lacks room for a second items buffer, each wave rebuilds half of it).
 enum KernelMode : int { kModeAll = 0, kModeBeam = 1, kModeBeamWide = 2, kModeBeamPair = 3 };
 
 }  // namespace nmsamd

@@ -60,6 +60,36 @@ __device__ __forceinline__ int checked_node(int id, int n, int* err) {
   return 0;
 }
 
+// The visited marks of one slot (hnsw.h:568-591: the reference's per-query
+// byte array with a version counter).  Bytes: one byte per node holding the
+// epoch of the query that marked it, the epoch bumped per query and the slot
+// cleared when it wraps.  Bits: one bit per node, the slot's first n / 8
+// bytes cleared when the query starts; eight times less memory per slot, so
+// the marks of every resident query stay in the device's caches.  A bit is set
+// with an atomic OR (neighbours of one step may share a word) and read at
+// device scope, so a read after the wave's own OR (which completes in L2)
+// cannot be served by a stale first-level cache line.
+struct Marks {
+  uint8_t* p;
+  uint8_t ep;
+  bool bits;
+  // the raw mark of node t >= 0: a word of 32 bits, or the byte
+  __device__ __forceinline__ int raw(int t) const {
+    if (bits)
+      return (int)__hip_atomic_load(reinterpret_cast<uint32_t*>(p) + (t >> 5), __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+    return (int)p[t];
+  }
+  __device__ __forceinline__ bool seen(int raw_mark, int t) const {
+    return bits ? (((uint32_t)raw_mark >> (t & 31)) & 1u) != 0u : raw_mark == (int)ep;
+  }
+  __device__ __forceinline__ bool seen(int t) const { return seen(raw(t), t); }
+  __device__ __forceinline__ void mark(int t) const {
+    if (bits) atomicOr(reinterpret_cast<uint32_t*>(p) + (t >> 5), 1u << (t & 31));
+    else p[t] = ep;
+  }
+};
+
 template <int FUNC>
 __device__ __forceinline__ float term(float q, float x) {
   if constexpr (FUNC == kL2Sqr16 || FUNC == kL2Sqr) {
@@ -987,7 +1017,7 @@ __device__ __forceinline__ void finish_query(const SearchArgs& a, const WaveLds&
 // the last item, truncation at the capacity and the first unexpanded item are
 // computed over the interleaving.
 template <int D, int FUNC, int ST>
-__device__ __forceinline__ void v1merge_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, uint8_t ep,
+__device__ __forceinline__ void v1merge_query(const SearchArgs& a, const WaveLds& w, const Marks& vis,
                               int qi, int k, int ef, int lane, int h, uint64_t ltm) {
   const DevIndex& ix = a.ix;
   const int C = max(ef, k);
@@ -1000,7 +1030,7 @@ __device__ __forceinline__ void v1merge_query(const SearchArgs& a, const WaveLds
 
   if (lane == 0) {
     A[0] = make_uint2(__float_as_uint(curd), (uint32_t)cur);
-    vis[cur] = ep;
+    vis.mark(cur);
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   int na = 1, ns = 0;  // items in A and in S
@@ -1095,7 +1125,7 @@ __device__ __forceinline__ void v1merge_query(const SearchArgs& a, const WaveLds
     nl += 1 + __popcll(__ballot(valid));
     lap(0);
     bool fresh = false;
-    if (valid) fresh = vis[t] != ep;
+    if (valid) fresh = !vis.seen(t);
     if (ix.has_dups) {
       w.cand[lane] = t;
       WSYNC();
@@ -1103,7 +1133,7 @@ __device__ __forceinline__ void v1merge_query(const SearchArgs& a, const WaveLds
         if (w.cand[j] == t) fresh = false;
       WSYNC();
     }
-    if (fresh) vis[t] = ep;
+    if (fresh) vis.mark(t);
     const uint64_t fm = __ballot(fresh);
     const int ncand = __popcll(fm);
     if (fresh) w.cand[__popcll(fm & ltm)] = t;
@@ -1727,7 +1757,7 @@ __device__ __forceinline__ int sort_merge(uint2* A, int n_, int C_, uint2* X, in
 }
 
 template <int D, int FUNC, int ST, int WIDE, bool PAIR>
-__device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, uint8_t ep,
+__device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w, const Marks& vis,
                            int qi, int k, int ef, int lane, int h, uint64_t ltm,
                            PairLink& link) {
   const DevIndex& ix = a.ix;
@@ -1752,7 +1782,7 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
   int n = 1;
   if (lane == 0) {
     A[0] = make_uint2(__float_as_uint(curd), (uint32_t)cur);
-    vis[cur] = ep;
+    vis.mark(cur);
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   int ce = 0;
@@ -1874,7 +1904,7 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
   auto load_marks = [&]() __attribute__((always_inline)) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
 #pragma unroll
-    for (int r = 0; r < 8; ++r) vn[r] = (int)vis[tn[r] >= 0 ? tn[r] : 0];
+    for (int r = 0; r < 8; ++r) vn[r] = vis.raw(tn[r] >= 0 ? tn[r] : 0);
   };
   for (;;) {
     WSYNC();
@@ -1939,14 +1969,14 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
         }
       }
 #pragma unroll
-      for (int r = 0; r < 8; ++r) vb[r] = (int)vis[t[r] >= 0 ? t[r] : 0];
+      for (int r = 0; r < 8; ++r) vb[r] = vis.raw(t[r] >= 0 ? t[r] : 0);
     }
     bool fresh[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
       const bool valid = r < erows && t[r] >= 0;
       nl += __popcll(__ballot(valid));
-      fresh[r] = valid && vb[r] != (int)ep;
+      fresh[r] = valid && !vis.seen(vb[r], t[r]);
     }
     nl += nsel;
     // a neighbour shared by two picks is taken once (LDS hash, linear
@@ -1983,7 +2013,7 @@ __device__ __forceinline__ void beam_query(const SearchArgs& a, const WaveLds& w
     }
 #pragma unroll
     for (int r = 0; r < 8; ++r)
-      if (fresh[r]) vis[t[r]] = ep;
+      if (fresh[r]) vis.mark(t[r]);
     int ncand = 0;
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
@@ -2182,7 +2212,7 @@ __device__ __forceinline__ void beam_helper(const SearchArgs& a, const WaveLds& 
 // sorted-array space holds the ef-heap in [0, ef] and the result queue
 // (sorted by (distance, internal id), the order KNNQueue pops in) after it.
 template <int D, int FUNC, int ST>
-__device__ __forceinline__ void old_query(const SearchArgs& a, const WaveLds& w, uint8_t* vis, uint8_t ep, int qi,
+__device__ __forceinline__ void old_query(const SearchArgs& a, const WaveLds& w, const Marks& vis, int qi,
                           int k, int ef, int lane, int h, uint64_t ltm, int64_t slot) {
   const DevIndex& ix = a.ix;
   unsigned long long nexp = 0, nd0 = 0, ndu = 1, nl = 0;
@@ -2222,7 +2252,7 @@ __device__ __forceinline__ void old_query(const SearchArgs& a, const WaveLds& w,
     bi[0] = cur;
     bs = 1;
     offer_result(curd, cur);
-    vis[cur] = ep;
+    vis.mark(cur);
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   for (;;) {
@@ -2242,7 +2272,7 @@ __device__ __forceinline__ void old_query(const SearchArgs& a, const WaveLds& w,
     const bool valid = t >= 0;
     nl += 1 + __popcll(__ballot(valid));
     bool fresh = false;
-    if (valid) fresh = vis[t] != ep;
+    if (valid) fresh = !vis.seen(t);
     if (ix.has_dups) {
       w.cand[lane] = t;
       WSYNC();
@@ -2250,7 +2280,7 @@ __device__ __forceinline__ void old_query(const SearchArgs& a, const WaveLds& w,
         if (w.cand[j] == t) fresh = false;
       WSYNC();
     }
-    if (fresh) vis[t] = ep;
+    if (fresh) vis.mark(t);
     const uint64_t fm = __ballot(fresh);
     const int ncand = __popcll(fm);
     if (fresh) w.cand[__popcll(fm & ltm)] = t;
@@ -2318,8 +2348,9 @@ __device__ __forceinline__ void old_query(const SearchArgs& a, const WaveLds& w,
 // query of the batch, each kernel carrying only its own loop's register
 // state; kModeBeamWide keeps a round's vector loads in flight together (one
 // wave per SIMD).  kModeBeamPair gives each query a block of two waves: the
-// first runs the search, the second computes the later rounds of each step's
-// distances and rebuilds half of the sorted array (beam_helper).
+// first runs the search, the second computes each step's distances (with the
+// deferred rebuild) or their later rounds and half of the rebuild
+// (beam_helper).
 template <int D, int FUNC, int ST, int MODE>
 __device__ __forceinline__ void search_body(const SearchArgs& a, int lds_wave_bytes) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -2339,7 +2370,7 @@ __device__ __forceinline__ void search_body(const SearchArgs& a, int lds_wave_by
   PairLink link{w.ctl, 0, w.items, w.cand};
   const int64_t slot = a.slot_base + (PAIR ? (int64_t)blockIdx.x
                                            : (int64_t)blockIdx.x * (blockDim.x >> 6) + wave);
-  uint8_t* vis = a.ix.visited + slot * a.ix.vstride;
+  uint8_t* vslot = a.ix.visited + slot * a.ix.vstride;
   const int dim = a.ix.dim;
   const int64_t nq = a.nq_dev ? *a.nq_dev : a.nq;
 
@@ -2353,19 +2384,30 @@ __device__ __forceinline__ void search_body(const SearchArgs& a, int lds_wave_by
     const int ef = a.ef_of ? a.ef_of[qi] : a.ef;
     const uint64_t t_start = __builtin_amdgcn_s_memrealtime();
 
-    // a fresh epoch for this slot's visited marks (hnsw.h:576-583)
-    int epi = 0;
-    if (lane == 0) epi = a.ix.epoch[slot];
-    epi = __shfl(epi, 0);
-    uint8_t ep = (uint8_t)(epi + 1);
-    if (ep == 0) {
-      uint4* v4 = reinterpret_cast<uint4*>(vis);
-      const int64_t nv4 = a.ix.vstride / 16;
+    // a fresh epoch for this slot's visited marks (hnsw.h:576-583), or bit
+    // marks cleared once the last query's ORs have completed
+    Marks vis{vslot, 0, a.vbits != 0};
+    if (vis.bits) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      uint4* v4 = reinterpret_cast<uint4*>(vslot);
+      const int64_t nv4 = (a.ix.n + 128) / 128;  // 32 marks per word, 4 words per store
       for (int64_t i = lane; i < nv4; i += 64) v4[i] = make_uint4(0, 0, 0, 0);
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      ep = 1;
+    } else {
+      int epi = 0;
+      if (lane == 0) epi = a.ix.epoch[slot];
+      epi = __shfl(epi, 0);
+      uint8_t ep = (uint8_t)(epi + 1);
+      if (ep == 0) {
+        uint4* v4 = reinterpret_cast<uint4*>(vslot);
+        const int64_t nv4 = a.ix.vstride / 16;
+        for (int64_t i = lane; i < nv4; i += 64) v4[i] = make_uint4(0, 0, 0, 0);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        ep = 1;
+      }
+      if (lane == 0) a.ix.epoch[slot] = ep;
+      vis.ep = ep;
     }
-    if (lane == 0) a.ix.epoch[slot] = ep;
 
     // the query, normalised in place for cosine (hnsw.h:486-497)
     const float* qg = a.queries + (int64_t)qi * dim;
@@ -2400,11 +2442,11 @@ __device__ __forceinline__ void search_body(const SearchArgs& a, int lds_wave_by
     if constexpr (MODE == kModeAll) {
       const bool use_old = a.algo == kAlgoOld || (a.algo == kAlgoHybrid && ef >= 1000);
       if (use_old)
-        old_query<D, FUNC, ST>(a, w, vis, ep, qi, k, ef, lane, h, ltm, slot);
+        old_query<D, FUNC, ST>(a, w, vis, qi, k, ef, lane, h, ltm, slot);
       else
-        v1merge_query<D, FUNC, ST>(a, w, vis, ep, qi, k, ef, lane, h, ltm);
+        v1merge_query<D, FUNC, ST>(a, w, vis, qi, k, ef, lane, h, ltm);
     } else {
-      beam_query<D, FUNC, ST, MODE == kModeBeamWide, PAIR>(a, w, vis, ep, qi, k, ef, lane, h, ltm,
+      beam_query<D, FUNC, ST, MODE == kModeBeamWide, PAIR>(a, w, vis, qi, k, ef, lane, h, ltm,
                                                           link);
     }
     WSYNC();

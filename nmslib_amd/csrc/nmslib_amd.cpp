@@ -108,6 +108,8 @@ struct nmsamd_index {
   int beam = 0;  // 0: exact V1Merge; > 0: beam mode width
   int beam_waves = 0;  // waves per beam query: 0 = by LDS footprint, 1 or 2
   int beam_gather = 0;  // one-wave beam kernel: 0 = by LDS footprint, 1 = plain, 2 = wide
+  int visited_marks = 0;  // 0 = by size (use_bit_marks), 1 = epoch bytes, 2 = bits
+  bool slots_bits = false;  // the last launch over d_visited left bit marks in it
   bool collect = false;
   // file header (hnsw.cc:779-788)
   bool loaded = false;
@@ -510,6 +512,15 @@ void ensure_slots(nmsamd_index* h, int slots) {
   hip_check(hipMemsetAsync(h->d_visited.p, 0, (size_t)slots * h->vstride, h->stream), "memset");
   hip_check(hipMemsetAsync(h->d_epoch.p, 0, (size_t)slots, h->stream), "memset");
   h->slots = slots;
+  h->slots_bits = false;
+}
+
+// Visited marks as bits (a query clears its slot's n / 8 bytes when it
+// starts) where that clearing is small next to the query's own mark probes
+// (about ef * maxM0), else epoch bytes (no clearing).
+bool use_bit_marks(const nmsamd_index* h, int max_ef) {
+  if (h->visited_marks) return h->visited_marks == 2;
+  return ((int64_t)h->n + 1) / 8 <= (int64_t)8 * std::max(max_ef, 1) * std::max<int64_t>(h->maxM0, 1);
 }
 
 // The reduced-precision copy the current storage traverses, converted on the
@@ -662,6 +673,7 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
     cap = std::max(cap, std::max(max_ef, max_k) + r);
   }
   const int ent = beam ? step_entries(h) : 128;
+  const bool vbits = use_bit_marks(h, max_ef);
   const int lds = lds_wave_bytes(cap, h->dstride, ent, beam);
   if ((size_t)lds > kLdsPerCu)
     fail(NMSAMD_EINVAL, "efSearch / k too large: the search array of " + std::to_string(cap) +
@@ -739,6 +751,16 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   } else {
     ensure_slots(h, slots);
     ix = dev_index(h);
+    // bit marks left in the slots would read as stale epochs: clear them
+    // before an epoch-byte launch (an epoch never equals a zero byte)
+    if (!vbits && h->slots_bits) {
+      const size_t words = (size_t)h->slots * h->vstride / 4;
+      for (size_t o = 0; o < words; o += (size_t)1 << 30)
+        hip_check(launch_fill_i32(reinterpret_cast<int*>(h->d_visited.p) + o, 0,
+                                  (int)std::min(words - o, (size_t)1 << 30), s), "fill");
+      h->slots_bits = false;
+    }
+    if (vbits) h->slots_bits = true;
     if (any_old) {
       h->d_oldk.ensure((size_t)slots * heap_cap);
       h->d_oldi.ensure((size_t)slots * heap_cap);
@@ -785,6 +807,7 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   a.old_heap_id = old_id;
   a.old_heap_cap = heap_cap;
   a.defer = defer ? 1 : 0;
+  a.vbits = vbits ? 1 : 0;
   hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
   hip_check(launch_search(a, blocks, wpb, lds_launch, mode, s), "search kernel launch");
   hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
@@ -1318,6 +1341,15 @@ int nmsamd_set_beam_gather(nmsamd_index* h, int gather) {
   });
 }
 
+int nmsamd_set_visited_marks(nmsamd_index* h, int marks) {
+  return guarded([&] {
+    if (!h) fail(NMSAMD_EINVAL, "null handle");
+    if (marks < 0 || marks > 2)
+      fail(NMSAMD_EINVAL, "visited marks must be 0 (by size), 1 (epoch bytes) or 2 (bits)");
+    h->visited_marks = marks;
+  });
+}
+
 int nmsamd_set_beam_waves(nmsamd_index* h, int waves) {
   return guarded([&] {
     if (!h) fail(NMSAMD_EINVAL, "null handle");
@@ -1424,6 +1456,7 @@ struct nmsamd_server {
   int storage = kF32, rerank = 0, beam = 0;
   int pair = 0;  // two waves per query (kModeBeamPair), from the LDS footprint
   int defer = 0;  // ... with the deferred rebuild (a second items buffer)
+  int vbits = 0;  // visited marks as bits
   DevIndex ix{};
   int slots_per_lane = 0;
   bool use_graph = false;
@@ -1467,6 +1500,7 @@ SearchArgs server_args(nmsamd_server* s, ServerLane& L) {
   a.algo = kAlgoV1Merge;
   a.slot_base = L.slot_base;
   a.defer = s->defer;
+  a.vbits = s->vbits;
   return a;
 }
 
@@ -1512,6 +1546,7 @@ int nmsamd_server_open(nmsamd_index* h, int lanes, int max_queries, int max_ef, 
     s->storage = h->storage;
     s->rerank = (h->storage != kF32 && h->rerank_mult > 0) ? h->rerank_mult : 0;
     s->beam = h->beam;
+    s->vbits = use_bit_marks(h, max_ef) ? 1 : 0;
     // the batch entry points' launch shape: two waves per query where the
     // footprint limits a CU to three queries (one query per block, LDS
     // padded so no fourth block fits), else one, wide where the footprint
