@@ -186,13 +186,14 @@ int nmsamd_set_beam_waves(nmsamd_index* h, int waves);
  * depend on it.  No reference counterpart (an autotuner knob). */
 int nmsamd_set_beam_gather(nmsamd_index* h, int gather);
 
-/* Visited marks of the batch entry points' searches (the reference keeps a
- * byte per node with a version counter, hnsw.h:568-591): 1 = epoch bytes (n
- * bytes per resident query, cleared when the epoch wraps), 2 = bits (n / 8
- * bytes, cleared when each query starts), 0 (default) = bits where clearing
- * n / 8 bytes is small next to about efSearch * maxM0 mark probes per query.
- * Results do not depend on it.  Servers choose at open by their max_ef.  No
- * reference counterpart (a tuning knob). */
+/* Visited marks of the beam-mode searches (the reference keeps a byte per
+ * node with a version counter, hnsw.h:568-591): 1 = epoch bytes (n bytes per
+ * resident query, cleared when the epoch wraps), 2 = bits (n / 8 bytes,
+ * cleared when each query starts), 0 (default) = bits where clearing n / 8
+ * bytes is small next to about efSearch * maxM0 mark probes per query.  The
+ * exact searches always take epoch bytes.  Results do not depend on it.
+ * Servers choose at open by their max_ef.  No reference counterpart (a tuning
+ * knob). */
 int nmsamd_set_visited_marks(nmsamd_index* h, int marks);
 
 /* Per-call counters of the last search (expansions, level-0 distance

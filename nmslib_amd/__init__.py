@@ -260,9 +260,10 @@ class FloatIndex:
         _check(load_library().nmsamd_set_beam_gather(self._h, int(gather)))
 
     def setVisitedMarks(self, marks: int = 0):
-        """Visited marks: 1 = epoch bytes, 2 = bits cleared per query, 0 =
-        bits where clearing n / 8 bytes is small next to efSearch * maxM0
-        probes.  Results do not depend on it."""
+        """Visited marks of beam mode: 1 = epoch bytes, 2 = bits cleared per
+        query, 0 = bits where clearing n / 8 bytes is small next to efSearch *
+        maxM0 probes (the exact searches keep epoch bytes).  Results do not
+        depend on it."""
         _check(load_library().nmsamd_set_visited_marks(self._h, int(marks)))
 
     def collectStats(self, on: bool = True):

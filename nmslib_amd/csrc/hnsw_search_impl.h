@@ -2385,8 +2385,9 @@ __device__ __forceinline__ void search_body(const SearchArgs& a, int lds_wave_by
     const uint64_t t_start = __builtin_amdgcn_s_memrealtime();
 
     // a fresh epoch for this slot's visited marks (hnsw.h:576-583), or bit
-    // marks cleared once the last query's ORs have completed
-    Marks vis{vslot, 0, a.vbits != 0};
+    // marks cleared once the last query's ORs have completed (beam mode only:
+    // the exact searches keep epoch bytes, their code free of the choice)
+    Marks vis{vslot, 0, MODE != kModeAll && a.vbits != 0};
     if (vis.bits) {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       uint4* v4 = reinterpret_cast<uint4*>(vslot);

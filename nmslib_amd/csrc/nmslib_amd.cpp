@@ -515,10 +515,12 @@ void ensure_slots(nmsamd_index* h, int slots) {
   h->slots_bits = false;
 }
 
-// Visited marks as bits (a query clears its slot's n / 8 bytes when it
-// starts) where that clearing is small next to the query's own mark probes
-// (about ef * maxM0), else epoch bytes (no clearing).
-bool use_bit_marks(const nmsamd_index* h, int max_ef) {
+// Visited marks of a beam search as bits (a query clears its slot's n / 8
+// bytes when it starts) where that clearing is small next to the query's own
+// mark probes (about ef * maxM0), else epoch bytes (no clearing).  The exact
+// searches (V1Merge, SearchOld) always take epoch bytes.
+bool use_bit_marks(const nmsamd_index* h, int max_ef, bool beam) {
+  if (!beam) return false;
   if (h->visited_marks) return h->visited_marks == 2;
   return ((int64_t)h->n + 1) / 8 <= (int64_t)8 * std::max(max_ef, 1) * std::max<int64_t>(h->maxM0, 1);
 }
@@ -673,7 +675,7 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
     cap = std::max(cap, std::max(max_ef, max_k) + r);
   }
   const int ent = beam ? step_entries(h) : 128;
-  const bool vbits = use_bit_marks(h, max_ef);
+  const bool vbits = use_bit_marks(h, max_ef, beam);
   const int lds = lds_wave_bytes(cap, h->dstride, ent, beam);
   if ((size_t)lds > kLdsPerCu)
     fail(NMSAMD_EINVAL, "efSearch / k too large: the search array of " + std::to_string(cap) +
@@ -715,7 +717,10 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   const int mode = !beam ? kModeAll : (pair ? kModeBeamPair : (wide ? kModeBeamWide : kModeBeam));
   int64_t max_blocks = (int64_t)h->cu_count * per_cu_run;
   const int64_t heap_cap = any_old ? std::max<int64_t>(65536, 64 * (int64_t)max_ef) : 0;
-  const int64_t vstride = ((int64_t)h->n + 1 + 255) / 256 * 256;
+  // a slot of the external scratch holds n + 1 epoch bytes, or with bit marks
+  // the (n + 128) / 128 16-byte words a query clears (hnsw_search_impl.h)
+  const int64_t vstride = vbits ? (((int64_t)h->n + 128) / 128 * 16 + 255) / 256 * 256
+                                : ((int64_t)h->n + 1 + 255) / 256 * 256;
   // visited marks take n bytes per slot: keep them under ~16 GiB, or within
   // the external scratch (256 B of counters, the epochs, then per slot its
   // marks and SearchOld heap)
@@ -1546,7 +1551,7 @@ int nmsamd_server_open(nmsamd_index* h, int lanes, int max_queries, int max_ef, 
     s->storage = h->storage;
     s->rerank = (h->storage != kF32 && h->rerank_mult > 0) ? h->rerank_mult : 0;
     s->beam = h->beam;
-    s->vbits = use_bit_marks(h, max_ef) ? 1 : 0;
+    s->vbits = use_bit_marks(h, max_ef, h->beam > 0) ? 1 : 0;
     // the batch entry points' launch shape: two waves per query where the
     // footprint limits a CU to three queries (one query per block, LDS
     // padded so no fourth block fits), else one, wide where the footprint
