@@ -156,7 +156,9 @@ def serve_bench(idx, q, forms, rates, duration_s, lanes=8, max_queries=1024, ef_
             log(f"[bench] serve {name} ef={'50..800' if not ef_fixed else ef_fixed} "
                 f"offered={rate} -> {rows[-1]}")
         srv.close()
-        ok = [r for r in rows if r["latency_ms_p99"] <= 5.0 and r["qps"] >= 0.97 * r["offered_qps"]]
+        # sustained: served at the rate the requests arrived (the Poisson sample
+        # of the offered rate) with p99 <= 5 ms
+        ok = [r for r in rows if r["latency_ms_p99"] <= 5.0 and r["qps"] >= 0.97 * r["arrival_qps"]]
         out.append({"form": name, "efSearch": f"{ef_fixed}" if ef_fixed else "50..800",
                     "k": "10" if ef_fixed else "1,10,100", "lanes": lanes,
                     "max_queries_per_batch": max_queries,

@@ -179,10 +179,13 @@ def run_poisson(server: Server, pool: np.ndarray, requests, keep_results: bool =
             if delay > 0:
                 time.sleep(min(delay, 0.001))
     span = max(r.done for r in requests) - requests[0].arrival if requests else 0.0
+    # the arrival process's own rate (a Poisson sample of the offered rate)
+    arrive = requests[-1].arrival - requests[0].arrival if total > 1 else 0.0
     lat = np.array([r.done - r.arrival for r in requests]) * 1000.0
     nqueries = sum(len(r.rows) for r in requests)
     return {"requests": total, "queries": int(nqueries),
             "qps": nqueries / span if span > 0 else 0.0,
+            "arrival_qps": nqueries / arrive if arrive > 0 else 0.0,
             "latency_ms_p50": float(np.percentile(lat, 50)) if total else 0.0,
             "latency_ms_p99": float(np.percentile(lat, 99)) if total else 0.0,
             "latency_ms_max": float(lat.max()) if total else 0.0}
