@@ -139,9 +139,10 @@ def serve_bench(idx, q, forms, rates, duration_s, lanes=8, max_queries=1024, ef_
     from nmslib_amd import serving
 
     out = []
-    for name, storage, rerank, beam in forms:
+    for name, storage, rerank, beam, waves in forms:
         idx.setPrecision(storage, rerank)
         idx.setBeam(beam)
+        idx.setBeamWaves(waves)
         idx.setQueryTimeParams({"algoType": "v1merge"})
         srv = serving.Server(idx, lanes=lanes, max_queries=max_queries, max_ef=max_ef,
                              max_k=max(ks), use_graph=True)
@@ -166,6 +167,7 @@ def serve_bench(idx, q, forms, rates, duration_s, lanes=8, max_queries=1024, ef_
                     "peak_qps": max(r["qps"] for r in rows), "rates": rows})
     idx.setPrecision("fp32", 0)
     idx.setBeam(0)
+    idx.setBeamWaves(0)
     return out
 
 
@@ -174,9 +176,10 @@ def single_query_latency(idx, q, forms, ef, n=32):
     the floor of any served request's latency at that operating point."""
     out = {}
     idx.setQueryTimeParams({"efSearch": ef, "algoType": "v1merge"})
-    for name, storage, rerank, beam in forms:
+    for name, storage, rerank, beam, waves in forms:
         idx.setPrecision(storage, rerank)
         idx.setBeam(beam)
+        idx.setBeamWaves(waves)
         idx.knnQueryBatchArrays(q[:1], 10)
         ms = []
         for i in range(n):
@@ -188,18 +191,26 @@ def single_query_latency(idx, q, forms, ef, n=32):
         log(f"[bench] single-query latency {name}: {out[name]}")
     idx.setPrecision("fp32", 0)
     idx.setBeam(0)
+    idx.setBeamWaves(0)
     return out
 
 
 FORMS = {
-    # name: (storage, rerank multiple of k, beam width)
-    "fp32-exact": ("fp32", 0, 0),
-    "fp32-beam4": ("fp32", 0, 4),
-    "bf16-beam4-rerank": ("bf16", 2, 4),
-    "fp32-beam8": ("fp32", 0, 8),
-    "fp32-beam16": ("fp32", 0, 16),
-    "bf16-beam16-rerank": ("bf16", 2, 16),
-    "fp8-beam16-rerank": ("fp8", 2, 16),
+    # name: (storage, rerank multiple of k, beam width, waves per query: 0 =
+    # by LDS footprint, 8 = the cooperative kernel)
+    "fp32-exact": ("fp32", 0, 0, 0),
+    "fp32-beam4": ("fp32", 0, 4, 0),
+    "bf16-beam4-rerank": ("bf16", 2, 4, 0),
+    "fp32-beam8": ("fp32", 0, 8, 0),
+    "fp32-beam16": ("fp32", 0, 16, 0),
+    "bf16-beam16-rerank": ("bf16", 2, 16, 0),
+    "fp8-beam16-rerank": ("fp8", 2, 16, 0),
+    "fp32-coop16": ("fp32", 0, 16, 8),
+    "fp32-coop32": ("fp32", 0, 32, 8),
+    "fp32-coop64": ("fp32", 0, 64, 8),
+    "bf16-coop16-rerank": ("bf16", 2, 16, 8),
+    "bf16-coop32-rerank": ("bf16", 2, 32, 8),
+    "bf16-coop64-rerank": ("bf16", 2, 64, 8),
 }
 
 
@@ -207,9 +218,10 @@ def measure(idx, q, gids, gd, form, args, dev, log_prefix):
     """Ladder to recall@10 >= 0.95, then a counters run and the timed steps."""
     import torch
 
-    storage, rerank, beam = FORMS[form]
+    storage, rerank, beam, waves = FORMS[form]
     idx.setPrecision(storage, rerank)
     idx.setBeam(beam)
+    idx.setBeamWaves(waves)
     nq, k = len(q), 10
     dq = torch.from_numpy(q).to(dev)
     dids = torch.empty((nq, k), dtype=torch.int32, device=dev)
@@ -410,7 +422,8 @@ def main():
     ap.add_argument("--ef", type=int, default=0, help="fixed efSearch (0: smallest of the ladder "
                     "reaching recall@10 >= 0.95)")
     ap.add_argument("--algo", default="v1merge")
-    ap.add_argument("--forms", default="fp32-exact,fp32-beam16,bf16-beam16-rerank",
+    ap.add_argument("--forms", default="fp32-exact,fp32-beam16,bf16-beam16-rerank,fp32-coop64,"
+                                       "bf16-coop64-rerank",
                     help=f"comma list of {sorted(FORMS)}; the headline is the fastest")
     ap.add_argument("--nq", type=int, default=10000)
     ap.add_argument("--no-sweep", dest="sweep", action="store_false",
@@ -418,9 +431,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve", default="5000,20000,50000,100000,200000,400000,800000",
                     help="offered query rates (q/s) of the serving run, '' to skip")
-    ap.add_argument("--serve-hi", default="20000,50000,80000,110000",
+    ap.add_argument("--serve-hi", default="10000,20000,50000,80000",
                     help="offered rates of the serving run at the headline efSearch")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--serve-forms", default="",
+                    help="comma list of serving form names to run (default: all)")
     ap.add_argument("--serve-lanes", type=int, default=8,
                     help="concurrent batch lanes of the serving run")
     ap.add_argument("--serve-batch", type=int, default=1024,
@@ -488,9 +503,10 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         form = args.forms.split(",")[-1]
-        storage, rerank, beam = FORMS[form]
+        storage, rerank, beam, waves = FORMS[form]
         idx.setPrecision(storage, rerank)
         idx.setBeam(beam)
+        idx.setBeamWaves(waves)
         idx.setQueryTimeParams({"efSearch": args.ef or 4736, "algoType": args.algo})
         ids, dists, counts = idx.knnQueryBatchArrays(q, 10)
         log(f"[bench] pmc pass: copy {nbytes} B, one {form} launch, recall "
@@ -529,17 +545,23 @@ def main():
     log(f"[bench] brute force {brute}")
 
     serve = None
-    if args.serve:
+    if args.serve or args.serve_hi:
         rates = [float(r) for r in args.serve.split(",") if r]
-        serve_forms = [("fp32 exact V1Merge", "fp32", 0, 0),
-                       ("fp32 beam16", "fp32", 0, 16),
-                       ("bf16 beam16 + fp32 rerank x2", "bf16", 2, 16)]
+        # (name, storage, rerank multiple, beam width, waves per query)
+        serve_forms = [("fp32 exact V1Merge", "fp32", 0, 0, 0),
+                       ("fp32 beam16", "fp32", 0, 16, 0),
+                       ("bf16 beam16 + fp32 rerank x2", "bf16", 2, 16, 0),
+                       ("fp32 coop64", "fp32", 0, 64, 8),
+                       ("bf16 coop64 + fp32 rerank x2", "bf16", 2, 64, 8)]
+        if args.serve_forms:
+            serve_forms = [f for f in serve_forms if f[0] in args.serve_forms.split(",")]
         mix = serve_bench(idx, q, serve_forms, rates, args.serve_seconds, args.serve_lanes,
-                          args.serve_batch)
+                          args.serve_batch) if rates else []
         # the recall@10 >= 0.95 operating point: every request at the headline
         # efSearch, k = 10
         hi_rates = [float(r) for r in args.serve_hi.split(",") if r]
-        hi = serve_bench(idx, q, serve_forms[1:], hi_rates, args.serve_seconds, args.serve_lanes,
+        hi = serve_bench(idx, q, [f for f in serve_forms if f[3] > 0], hi_rates, args.serve_seconds,
+                         args.serve_lanes,
                          args.serve_batch, ef_fixed=head["efSearch"], max_ef=head["efSearch"],
                          ks=(10,)) if hi_rates else []
         alone = single_query_latency(idx, q, serve_forms, head["efSearch"])

@@ -164,18 +164,21 @@ int nmsamd_index_info(const nmsamd_index* h, int64_t* out8);
 int nmsamd_set_precision(nmsamd_index* h, int storage, int rerank_mult);
 
 /* Search mode: 0 = the reference's SearchV1Merge replayed exactly (default);
- * 1..16 = beam mode, that many unexpanded items expanded per step (at most
- * 512 / maxM0) with ties ordered by internal id (no reference counterpart;
- * same array, capacity and stop rule, fewer serial steps).  Beam mode runs
- * every query of a batch, whatever its ef (the hybrid switch to SearchOld at
- * ef >= 1000 does not apply); algoType=old still runs SearchOld. */
+ * 1..64 = beam mode, that many unexpanded items expanded per step (at most
+ * 512 / maxM0, or 2048 / maxM0 on the cooperative kernel, which widths above
+ * 16 take) with ties ordered by internal id (no reference counterpart; same
+ * array, capacity and stop rule, fewer serial steps).  Beam mode runs every
+ * query of a batch, whatever its ef (the hybrid switch to SearchOld at ef >=
+ * 1000 does not apply); algoType=old still runs SearchOld. */
 int nmsamd_set_beam(nmsamd_index* h, int beam);
 
 /* Waves per query in beam mode: 1, 2 (a block of two waves per query: the
  * second computes each step's distances while the first rebuilds the sorted
- * array from the last merge; at most three queries per CU), or 0 (default) =
- * two where the LDS footprint limits a CU to three queries.  Results do not
- * depend on it.  No reference counterpart. */
+ * array from the last merge; at most three queries per CU), 8 (the
+ * cooperative kernel: a block of eight waves per query sharing every phase
+ * of a step, one query per CU, the low-latency form; also taken by beam
+ * widths above 16), or 0 (default) = two where the LDS footprint limits a CU
+ * to three queries.  Results do not depend on it.  No reference counterpart. */
 int nmsamd_set_beam_waves(nmsamd_index* h, int waves);
 
 /* Vector loads in flight in the one-wave beam kernels: 1 = the plain kernel
@@ -278,7 +281,8 @@ int nmsamd_server_poll(nmsamd_server* s, int lane);
 /* Copies the completed batch's results out ([nq][max_k] rows, counts[nq])
  * and frees the lane. */
 int nmsamd_server_fetch(nmsamd_server* s, int lane, int32_t* ids, float* dists, int32_t* counts);
-/* lanes, max_queries, max_ef, max_k, slots per lane, waves per block, graph, wide */
+/* lanes, max_queries, max_ef, max_k, slots per lane, waves per block, graph,
+ * beam kernel (0 one wave, 1 one wave wide, 2 two waves, 3 cooperative) */
 int nmsamd_server_info(const nmsamd_server* s, int64_t* out8);
 int nmsamd_server_close(nmsamd_server* s);
 

@@ -242,15 +242,17 @@ class FloatIndex:
         _check(load_library().nmsamd_set_precision(self._h, code, int(rerank_mult)))
 
     def setBeam(self, beam: int = 0):
-        """0: exact V1Merge (the reference's results); 1..16: beam mode
-        (expansions per step, capped at 512 // maxM0)."""
+        """0: exact V1Merge (the reference's results); 1..64: beam mode
+        (expansions per step, capped at 512 // maxM0, or 2048 // maxM0 on
+        the cooperative kernel that widths above 16 take)."""
         _check(load_library().nmsamd_set_beam(self._h, int(beam)))
 
     def setBeamWaves(self, waves: int = 0):
         """Waves per beam query: 1, 2 (the second wave computes each step's
         distances while the first rebuilds the sorted array from the last
-        merge) or 0 = chosen from the LDS footprint.  Results do not depend
-        on it."""
+        merge), 8 (the cooperative kernel: one query per CU, every phase of
+        a step shared by eight waves; the low-latency form) or 0 = chosen
+        from the LDS footprint.  Results do not depend on it."""
         _check(load_library().nmsamd_set_beam_waves(self._h, int(waves)))
 
     def setBeamGather(self, gather: int = 0):
@@ -284,9 +286,13 @@ class FloatIndex:
         of the last search."""
         out = np.zeros(8, np.float64)
         _check(load_library().nmsamd_last_phase_us(self._h, _ptr(out)))
-        if self.lastLaunch()["mode"] == "v1merge/old":
+        mode = self.lastLaunch()["mode"]
+        if mode == "v1merge/old":
             keys = ("pick_adjacency", "visited", "distances", "admit_predict", "sort",
                     "rank_search", "merge", "advance")
+        elif mode == "beam (cooperative)":
+            keys = ("pick", "rows_marks", "distances_admit", "rank_new", "rank_array",
+                    "records", "rebuild", "advance")
         else:
             keys = ("pick", "adjacency_visited", "distances", "filter", "sort_predict",
                     "rank_search", "rebuild", "advance")
@@ -297,10 +303,11 @@ class FloatIndex:
         out = np.zeros(5, np.int64)
         _check(load_library().nmsamd_last_launch(self._h, _ptr(out)))
         mode = int(out[0])
-        kernel = {2: "hnsw_search_kernel_wide", 3: "hnsw_search_kernel_pair"}.get(
-            mode, "hnsw_search_kernel")
+        kernel = {2: "hnsw_search_kernel_wide", 3: "hnsw_search_kernel_pair",
+                  4: "hnsw_search_kernel_coop"}.get(mode, "hnsw_search_kernel")
         return {"kernel": kernel,
-                "mode": ("v1merge/old", "beam", "beam (wide)", "beam (two waves)")[mode],
+                "mode": ("v1merge/old", "beam", "beam (wide)", "beam (two waves)",
+                         "beam (cooperative)")[mode],
                 "blocks": int(out[1]),
                 "queries_per_block": int(out[2]), "lds_bytes_per_query": int(out[3]),
                 "entries_per_step": int(out[4])}

@@ -4,9 +4,11 @@ The knobs of the device search (none changes which results a form returns,
 except storage and rerank, whose recall is measured):
 
 * beam width W: expansions per step, 0 = the reference's exact V1Merge,
-  1..16 = beam mode;
-* waves per query in beam mode: 1, or 2 (the second wave computes each
-  step's distances while the first rebuilds the sorted array);
+  1..64 = beam mode;
+* waves per query in beam mode: 1, 2 (the second wave computes each step's
+  distances while the first rebuilds the sorted array) or 8 (the
+  cooperative kernel: one query per CU, every phase of a step shared by eight
+  waves, the only kernel for widths above 16);
 * gather width of the one-wave kernels: plain (two waves per SIMD, one FP32 /
   two BF16 / four FP8 vectors per lane pair in flight) or wide (one wave per
   SIMD, two / five / eight);
@@ -41,6 +43,9 @@ STAGE1 = [
     ("bf16-beam8-rerank2", "bf16", 2, 8, 0, 0),
     ("bf16-beam16-rerank2", "bf16", 2, 16, 0, 0),
     ("fp8-beam8-rerank2", "fp8", 2, 8, 0, 0),
+    ("fp32-coop32", "fp32", 0, 32, 8, 0),
+    ("bf16-coop32-rerank2", "bf16", 2, 32, 8, 0),
+    ("bf16-coop64-rerank2", "bf16", 2, 64, 8, 0),
 ]
 # kept for callers that pass the old five forms explicitly
 DEFAULT_FORMS = [f[:4] for f in STAGE1]
@@ -71,15 +76,16 @@ def _apply(index, form):
 
 def _variants(form):
     """Stage 2: the same form with explicit waves / gather / rerank width."""
-    name, storage, rerank, beam, _, _ = form
+    name, storage, rerank, beam, waves, _ = form
     out = []
-    if beam > 0:
+    if 0 < beam <= 16 and waves != 8:
         out.append((f"{name}-1wave-plain", storage, rerank, beam, 1, 1))
         out.append((f"{name}-1wave-wide", storage, rerank, beam, 1, 2))
         out.append((f"{name}-2waves", storage, rerank, beam, 2, 0))
+        out.append((f"{name}-coop", storage, rerank, beam, 8, 0))
     if storage != "fp32":
         for r in (1, 4):
-            out.append((f"{name.rsplit('-rerank', 1)[0]}-rerank{r}", storage, r, beam, 0, 0))
+            out.append((f"{name.rsplit('-rerank', 1)[0]}-rerank{r}", storage, r, beam, waves, 0))
     return out
 
 

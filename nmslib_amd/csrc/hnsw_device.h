@@ -119,7 +119,11 @@ enum AlgoMode : int { kAlgoHybrid = 0, kAlgoOld = 1, kAlgoV1Merge = 2 };
 // with two waves per query (a block of two waves: the second computes each
 // step's distances, and the first rebuilds the sorted array meanwhile or, where
 // LDS lacks room for a second items buffer, each wave rebuilds half of it).
-enum KernelMode : int { kModeAll = 0, kModeBeam = 1, kModeBeamWide = 2, kModeBeamPair = 3 };
+enum KernelMode : int {
+  kModeAll = 0, kModeBeam = 1, kModeBeamWide = 2, kModeBeamPair = 3,
+  kModeBeamCoop = 4  // one query per block of kCoopWaves waves (beam up to 64)
+};
+constexpr int kCoopWaves = 8;
 
 }  // namespace nmsamd
 #endif

@@ -2203,6 +2203,415 @@ __device__ __forceinline__ void beam_helper(const SearchArgs& a, const WaveLds& 
 }
 
 // ---------------------------------------------------------------------------
+// Cooperative beam search (kModeBeamCoop; an extension with no reference
+// counterpart): one query per block of kCoopWaves waves, each step split
+// across them between workgroup barriers.  The step is beam_query's -- the
+// first W unexpanded items of the array expanded together, the snapshot
+// admission rule, the merge truncated at max(ef, k) -- so the results equal
+// the beam model's:
+//   1. picks and the first unexpanded item left (wave 0);
+//   2. adjacency rows, visited marks and compaction, the entries split across
+//      all threads: a mark is a bit set by an atomic OR that returns the old
+//      word, so the probe, the mark and the dedupe across the picks' lists are
+//      one operation (two copies of an id: one sees the bit clear);
+//   3. distances (each wave a slice of the candidates, wide gathers) and the
+//      snapshot admission, items placed through an LDS counter;
+//   4. the rank of every new item among the new items (one item per thread:
+//      keys are distinct (distance, id) pairs) and in the array (binary
+//      search), the merge records, then the rebuild walk in rounds of 64
+//      chunks, 8 per wave, every read of a round before any of its writes (a
+//      position reads at most m positions below itself, so a round only reads
+//      chunks that no earlier round has written).
+// A whole CU serves one query, and beam widths up to 64 shorten the serial
+// chain: this is the low-latency form.  Visited marks are always bits.
+// The m new items of a cooperative step, items[0..m), sorted by (distance,
+// id) into X: wave w sorts the run items[64 R w, 64 R (w + 1)) in registers
+// (bitonic_sort) and writes it back; then an item's rank is its index in its
+// run plus, for every other run, the number of that run's items before it
+// (binary search; keys are distinct (distance, id) pairs).  Ends with a
+// workgroup barrier.
+template <int R>
+__device__ __forceinline__ void coop_sort_new(uint2* items, uint2* X, int m, int wave, int lane) {
+  constexpr int RUN = 64 * R;
+  const int b0 = wave * RUN;
+  float k[R];
+  uint32_t id[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int j = b0 + r * 64 + lane;
+    if (j < m) {
+      const uint2 v = items[j];
+      k[r] = __uint_as_float(v.x);
+      id[r] = v.y;
+    } else {
+      k[r] = kInf();
+      id[r] = 0xffffffffu;
+    }
+  }
+  if (b0 < m) bitonic_sort<R>(k, id, lane);
+  WSYNC();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int j = b0 + r * 64 + lane;
+    if (j < m) items[j] = make_uint2(__float_as_uint(k[r]), id[r]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int j = b0 + r * 64 + lane;
+    if (j < m) {
+      int rank = r * 64 + lane;
+      for (int b = 0; b < kCoopWaves; ++b) {
+        const int s0 = b * RUN;
+        if (b == wave || s0 >= m) continue;
+        int lo = s0, hi = min(m, s0 + RUN);
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          const uint2 o = items[mid];
+          if (kid_less(__uint_as_float(o.x), o.y, k[r], id[r])) lo = mid + 1; else hi = mid;
+        }
+        rank += lo - s0;
+      }
+      X[rank] = make_uint2(__float_as_uint(k[r]), id[r]);
+    }
+  }
+  __syncthreads();
+}
+
+template <int D, int FUNC, int ST>
+__device__ __forceinline__ void coop_query(const SearchArgs& a, const WaveLds& w, const Marks& vis,
+                                           int qi, int k, int ef, int wave, int lane, int h,
+                                           uint64_t ltm) {
+  constexpr int NT = kCoopWaves * kWave;
+  constexpr int RMAX = 4;  // entries per thread and step: W * maxM0 <= 2048
+  const DevIndex& ix = a.ix;
+  const int tid = wave * kWave + lane;
+  const int C = max(ef, k);
+  const int W = max(1, min(a.beam, a.ent / ix.maxM0));
+  uint2* A = w.A;
+  int* ctl = w.ctl;
+  int* picks = reinterpret_cast<int*>(w.pfa);  // [W <= 64] (pfa, pfq, pfpos: 104 words)
+  int* wcnt = picks + 64;                      // [kCoopWaves] per-wave counts of a pick round
+  int* lrank = w.cand;                         // after the distances: ranks in A of the new items
+  uint2* X = w.items2;                         // the step's new items, sorted
+  uint32_t* vw = reinterpret_cast<uint32_t*>(vis.p);
+  unsigned long long nexp = 0, nd0 = 0, ndu = 1, nl = 0;
+
+  if (wave == 0) {
+    int cur = ix.enter;
+    float curd = dist_node<D, FUNC, ST>(w.qs, ix, cur, h);
+    greedy_descent<D, FUNC, ST>(ix, w.qs, lane, h, cur, curd, ndu, nl);
+    if (lane == 0) {
+      A[0] = make_uint2(__float_as_uint(curd), (uint32_t)cur);
+      vis.mark(cur);
+      ctl[4] = cur;
+      ctl[6] = 0;
+    }
+  }
+  __syncthreads();
+  const int fallback = ctl[4];
+  int n = 1, ce = 0;
+  // per-phase device time (100 MHz ticks, taken by thread 0 into ctl[10..18)
+  // when counters are collected): picks, rows + marks, distances +
+  // admission, ranks among the new items, ranks in the array, merge records,
+  // rebuild, advance
+  const bool timed = a.stats != nullptr && tid == 0;
+  uint64_t tmark = 0;
+  if (timed) {
+    for (int p = 0; p < 8; ++p) ctl[10 + p] = 0;
+    tmark = __builtin_amdgcn_s_memrealtime();
+  }
+  auto lap = [&](int p) __attribute__((always_inline)) {
+    if (timed) {
+      const uint64_t now = __builtin_amdgcn_s_memrealtime();
+      ctl[10 + p] += (int)(now - tmark);
+      tmark = now;
+    }
+  };
+  for (;;) {
+    // 1. the first W unexpanded items below min(n, ef), in position order,
+    // marked expanded, and the first unexpanded item left (pa, n if none):
+    // rounds of 512 positions from ce, one chunk of 64 per wave, the order of
+    // an unexpanded item among those found taken from the waves' counts
+    const int lim = min(n, ef);
+    if (tid == 0) {
+      ctl[2] = 0;  // picks
+      ctl[7] = n;  // pa
+      ctl[0] = 0;  // candidates
+      ctl[1] = 0;  // admitted items
+    }
+    int found = 0;  // unexpanded items in the rounds before this one
+    for (int base = ce; base < n; base += NT) {
+      const int i = base + tid;
+      const bool un = i < n && !(A[i].y & kUsedBit);
+      const uint64_t um = __ballot(un);
+      if (lane == 0) wcnt[wave] = __popcll(um);
+      __syncthreads();
+      int before = 0, total = 0;
+#pragma unroll
+      for (int v = 0; v < kCoopWaves; ++v) {
+        const int c = wcnt[v];
+        before += v < wave ? c : 0;
+        total += c;
+      }
+      const int r = found + before + __popcll(um & ltm);
+      const bool pick = un && i < lim && r < W;
+      if (pick) {
+        const uint32_t y = A[i].y;
+        A[i].y = y | kUsedBit;
+        picks[r] = (int)y;
+      }
+      // one LDS atomic per wave: its picks, and its first unexpanded item left
+      const uint64_t pm = __ballot(pick), lm = __ballot(un && !pick);
+      if (lane == 0) {
+        if (pm) atomicAdd(&ctl[2], __popcll(pm));
+        if (lm) atomicMin(&ctl[7], base + wave * kWave + __ffsll((long long)lm) - 1);
+      }
+      found += total;
+      __syncthreads();
+      if (ctl[7] < n) break;  // every later item is past lim or past the W picks
+    }
+    if (wave == 0) nl += ctl[2];
+    __syncthreads();
+    lap(0);
+    const int nsel = ctl[2];
+    if (nsel == 0) break;
+    nexp += nsel;
+    const float topk = akey(A, n - 1);
+    const int nsnap = n;
+    const int pa = ctl[7];
+
+    // 2. rows, marks and fresh entries: entry e = tid + NT r of the nsel
+    // lists of maxM0 slots; every row load, then every mark, in flight at once
+    const int nent = nsel * ix.maxM0;
+    const int R = (nent + NT - 1) / NT;
+    int t[RMAX];
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r) {
+      const int e = tid + NT * r;
+      t[r] = -1;
+      if (r < R && e < nent) {
+        const int li = e / ix.maxM0;
+        const int sl = e - li * ix.maxM0;
+        t[r] = ix.l0[(int64_t)checked_node(picks[li], (int)ix.n, a.err) * ix.l0stride + sl];
+      }
+    }
+    uint32_t old[RMAX];
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r) {
+      old[r] = 0xffffffffu;
+      if (t[r] >= 0) old[r] = atomicOr(vw + (t[r] >> 5), 1u << (t[r] & 31));
+    }
+    int lnl = 0;
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r) {
+      if (r < R) {
+        lnl += __popcll(__ballot(t[r] >= 0));
+        const bool fresh = t[r] >= 0 && !((old[r] >> (t[r] & 31)) & 1u);
+        const uint64_t fm = __ballot(fresh);
+        int base = 0;
+        if (lane == 0 && fm) base = atomicAdd(&ctl[0], __popcll(fm));
+        base = __shfl(base, 0);
+        if (fresh) w.cand[base + __popcll(fm & ltm)] = t[r];
+      }
+    }
+    if (lane == 0 && lnl) atomicAdd(&ctl[6], lnl);
+    __syncthreads();
+    lap(1);
+    const int ncand = ctl[0];
+    nd0 += ncand;
+
+    // 3. distances of a slice of the candidates per wave, and the admission
+    const int per = (ncand + kCoopWaves - 1) / kCoopWaves;
+    const int s0 = min(ncand, wave * per), s1 = min(ncand, s0 + per);
+    dist_cands<D, FUNC, ST, 1>(w.qs, ix, w.cand + s0, w.cdist + s0, s1 - s0, lane, h, fallback,
+                               a.err);
+    WSYNC();
+    for (int b = s0; b < s1; b += 64) {
+      const int s = b + lane;
+      const float dj = s < s1 ? w.cdist[s] : 0.f;
+      const bool keep = s < s1 && (dj < topk || nsnap < ef);
+      const uint64_t km = __ballot(keep);
+      int base = 0;
+      if (lane == 0 && km) base = atomicAdd(&ctl[1], __popcll(km));
+      base = __shfl(base, 0);
+      if (keep)
+        w.items[base + __popcll(km & ltm)] = make_uint2(__float_as_uint(dj), (uint32_t)w.cand[s]);
+    }
+    __syncthreads();
+    lap(2);
+    const int m = ctl[1];
+    if (m == 0) {
+      ce = pa;
+      continue;
+    }
+
+    // 4. ranks among the new items (sorted into X), then in the array
+    if (m <= 512) coop_sort_new<1>(w.items, X, m, wave, lane);
+    else if (m <= 1024) coop_sort_new<2>(w.items, X, m, wave, lane);
+    else coop_sort_new<4>(w.items, X, m, wave, lane);
+    lap(3);
+    for (int j = tid; j < m; j += NT) {
+      const uint2 v = X[j];
+      lrank[j] = lower_count_kid(A, n, __uint_as_float(v.x), v.y);
+    }
+    __syncthreads();
+    lap(4);
+    const int nout = min(C, n + m);
+    const int P0 = lrank[0];
+    if (P0 < nout) {
+      const int clo = P0 >> 6, chi = (nout - 1) >> 6;
+      for (int c = tid; c <= chi - clo; c += NT) w.meta[c] = make_uint4(0u, 0u, 0u, 0u);
+      __syncthreads();
+      for (int j = tid; j < m; j += NT) {
+        const int p = lrank[j] + j;
+        if (p < nout)
+          atomicOr(reinterpret_cast<unsigned long long*>(&w.meta[(p >> 6) - clo]), 1ull << (p & 63));
+      }
+      __syncthreads();
+      if (wave == 0) {
+        // new items below each chunk: an exclusive prefix over the chunks'
+        // counts, 64 chunks at a time, bit-sliced through ballots
+        const uint64_t below_lane = (1ull << lane) - 1;
+        int base = 0;
+        for (int g = 0; g <= chi - clo; g += 64) {
+          const int c = g + lane;
+          const bool in = c <= chi - clo;
+          const uint4 e = in ? w.meta[c] : make_uint4(0u, 0u, 0u, 0u);
+          const int cnt = __popc(e.x) + __popc(e.y);
+          int excl = base, tot = 0;
+#pragma unroll
+          for (int b = 0; b < 7; ++b) {
+            const uint64_t mb = __ballot((cnt >> b) & 1);
+            excl += __popcll(mb & below_lane) << b;
+            tot += __popcll(mb) << b;
+          }
+          if (in) w.meta[c].z = (uint32_t)excl;
+          base += tot;
+        }
+      }
+      __syncthreads();
+      lap(5);
+      // the rebuild: rounds of 64 chunks from the top, 8 per wave
+      for (int c0 = chi; c0 >= clo; c0 -= 64) {
+        uint2 val[8];
+        int pos[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int cc = c0 - 8 * wave - u;
+          const bool okc = cc >= clo;
+          const uint4 md = w.meta[(okc ? cc : clo) - clo];
+          const int p = (cc << 6) + lane;
+          const int bl = (int)md.z + (int)__builtin_amdgcn_mbcnt_hi(
+                                         md.y, __builtin_amdgcn_mbcnt_lo(md.x, 0u));
+          const bool mine = ((lane < 32 ? md.x : md.y) >> (lane & 31)) & 1u;
+          const bool ok = okc && p < nout;
+          const uint2* src = mine ? X + bl : A + (p - bl);
+          val[u] = *(ok ? src : A);
+          pos[u] = ok ? p : -1;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (pos[u] >= 0) A[pos[u]] = val[u];
+        __syncthreads();
+      }
+    }
+    lap(6);
+    // the first unexpanded item of the merged array: the old one left at pa
+    // (moved up by the new items before it) or the smallest new item
+    if (tid == 0) {
+      int pan = nout;
+      if (pa < n) {
+        int lo = 0, hi = m;  // new items with a rank <= pa (ranks are nondecreasing)
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (lrank[mid] <= pa) lo = mid + 1; else hi = mid;
+        }
+        pan = pa + lo;
+      }
+      ctl[9] = min(nout, min(pan, P0));
+    }
+    __syncthreads();
+    lap(7);
+    n = nout;
+    ce = ctl[9];
+  }
+  if (timed)
+    for (int p = 0; p < 8; ++p) atomicAdd(&a.stats->phase_ticks[p], (unsigned long long)ctl[10 + p]);
+  if (wave == 0) {
+    const QueryCounters c{nexp, nd0, ndu, nl + (unsigned long long)ctl[6], 0};
+    finish_query<D, FUNC, ST>(a, w, n, qi, k, lane, h, c);
+  }
+}
+
+// One query per block of kCoopWaves waves (coop_query).
+template <int D, int FUNC, int ST>
+__device__ __forceinline__ void coop_body(const SearchArgs& a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NT = kCoopWaves * kWave;
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int h = lane & 1;
+  const uint64_t ltm = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const WaveLds w = carve(smem, a.cap, a.ix.dstride, a.ent);
+  const int64_t slot = a.slot_base + (int64_t)blockIdx.x;
+  uint8_t* vslot = a.ix.visited + slot * a.ix.vstride;
+  const Marks vis{vslot, 0, true};
+  const int dim = a.ix.dim;
+  const int64_t nq = a.nq_dev ? *a.nq_dev : a.nq;
+  for (;;) {
+    if (threadIdx.x == 0) w.ctl[3] = atomicAdd(a.work, 1);
+    __syncthreads();
+    int qi = w.ctl[3];
+    if (qi >= nq) break;
+    if (a.order) qi = a.order[qi];
+    const int k = a.k_of ? a.k_of[qi] : a.k;
+    const int ef = a.ef_of ? a.ef_of[qi] : a.ef;
+    const uint64_t t_start = __builtin_amdgcn_s_memrealtime();
+    // clear the slot's bit marks (the last query's ORs have completed); the
+    // query into LDS, normalised in place for cosine (hnsw.h:486-497)
+    uint4* v4 = reinterpret_cast<uint4*>(vslot);
+    const int64_t nv4 = (a.ix.n + 128) / 128;
+    for (int64_t i = threadIdx.x; i < nv4; i += NT) v4[i] = make_uint4(0, 0, 0, 0);
+    const float* qg = a.queries + (int64_t)qi * dim;
+    for (int i = threadIdx.x; i < a.ix.dstride; i += NT) w.qs[i] = i < dim ? qg[i] : 0.f;
+    __syncthreads();  // with its release fence: the clearing stores are in L2 before any OR
+    if (wave == 0) {
+      if constexpr (FUNC == kNormCos) {
+        float inv = 0.f;
+        if (lane == 0) {
+          float sum = 0.f;
+          for (int i = 0; i < dim; ++i) sum = sum + w.qs[i] * w.qs[i];
+          inv = sum != 0.0f ? 1.0f / sqrtf(sum) : 0.f;
+        }
+        inv = __shfl(inv, 0);
+        if (inv != 0.f)
+          for (int i = lane; i < dim; i += 64) w.qs[i] = w.qs[i] * inv;
+        WSYNC();
+      }
+      if constexpr (ST == kBF16) make_query_bf16(w.qs, a.ix.dstride, lane);
+    }
+    __syncthreads();
+    if (k <= 0 || a.ix.n == 0) {
+      if (wave == 0) {
+        for (int p = lane; p < k; p += 64) {
+          a.out_ids[(int64_t)qi * a.out_stride + p] = -1;
+          a.out_dists[(int64_t)qi * a.out_stride + p] = 0.f;
+        }
+        if (lane == 0) a.out_counts[qi] = 0;
+      }
+      __syncthreads();
+      continue;
+    }
+    coop_query<D, FUNC, ST>(a, w, vis, qi, k, ef, wave, lane, h, ltm);
+    __syncthreads();
+    if (a.q_ticks && threadIdx.x == 0)
+      a.q_ticks[qi] = (uint32_t)(__builtin_amdgcn_s_memrealtime() - t_start);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // SearchOld at level 0 (hnsw_distfunc_opt.cc:99-148): a candidate heap
 // ordered by -distance (unbounded, in this slot's HBM scratch), a result heap
 // of at most ef items and the k-item result queue.  Admission of each fresh
@@ -2478,12 +2887,20 @@ __global__ void __launch_bounds__(2 * kWave) __attribute__((amdgpu_waves_per_eu(
   search_body<D, FUNC, ST, kModeBeamPair>(a, lds_wave_bytes);
 }
 
+// kModeBeamCoop: one query per block of kCoopWaves waves, two waves per SIMD
+template <int D, int FUNC, int ST>
+__global__ void __launch_bounds__(kCoopWaves * kWave) __attribute__((amdgpu_waves_per_eu(2)))
+    hnsw_search_kernel_coop(SearchArgs a, int) {
+  coop_body<D, FUNC, ST>(a);
+}
+
 using KernelFn = void (*)(SearchArgs, int);
 
 template <int D, int FUNC, int ST, int MODE>
 KernelFn kernel_of() {
   if constexpr (MODE == kModeBeamWide) return hnsw_search_kernel_wide<D, FUNC, ST>;
   else if constexpr (MODE == kModeBeamPair) return hnsw_search_kernel_pair<D, FUNC, ST>;
+  else if constexpr (MODE == kModeBeamCoop) return hnsw_search_kernel_coop<D, FUNC, ST>;
   else return hnsw_search_kernel<D, FUNC, ST, MODE>;
 }
 

@@ -14,7 +14,8 @@ using KernelFn = void (*)(SearchArgs, int);
   KernelFn pick_kernel_st##S##_m0(int dim, int func);  \
   KernelFn pick_kernel_st##S##_m1(int dim, int func);  \
   KernelFn pick_kernel_st##S##_m2(int dim, int func);  \
-  KernelFn pick_kernel_st##S##_m3(int dim, int func);
+  KernelFn pick_kernel_st##S##_m3(int dim, int func);  \
+  KernelFn pick_kernel_st##S##_m4(int dim, int func);
 NMS_DECL(0)
 NMS_DECL(1)
 NMS_DECL(2)
@@ -22,11 +23,14 @@ NMS_DECL(2)
 
 static KernelFn pick_kernel(int storage, int dim, int func, int mode) {
   using Pick = KernelFn (*)(int, int);
-  static const Pick table[3][4] = {
-      {pick_kernel_st0_m0, pick_kernel_st0_m1, pick_kernel_st0_m2, pick_kernel_st0_m3},
-      {pick_kernel_st1_m0, pick_kernel_st1_m1, pick_kernel_st1_m2, pick_kernel_st1_m3},
-      {pick_kernel_st2_m0, pick_kernel_st2_m1, pick_kernel_st2_m2, pick_kernel_st2_m3}};
-  if (storage < 0 || storage > 2 || mode < 0 || mode > 3) return nullptr;
+  static const Pick table[3][5] = {
+      {pick_kernel_st0_m0, pick_kernel_st0_m1, pick_kernel_st0_m2, pick_kernel_st0_m3,
+       pick_kernel_st0_m4},
+      {pick_kernel_st1_m0, pick_kernel_st1_m1, pick_kernel_st1_m2, pick_kernel_st1_m3,
+       pick_kernel_st1_m4},
+      {pick_kernel_st2_m0, pick_kernel_st2_m1, pick_kernel_st2_m2, pick_kernel_st2_m3,
+       pick_kernel_st2_m4}};
+  if (storage < 0 || storage > 2 || mode < 0 || mode > 4) return nullptr;
   return table[storage][mode](dim, func);
 }
 
@@ -35,17 +39,18 @@ hipError_t launch_search(const SearchArgs& a, int blocks, int waves_per_block, i
   KernelFn fn = pick_kernel(a.storage, a.ix.dim, a.ix.func, mode);
   if (!fn) return hipErrorInvalidValue;
   if (waves_per_block < 1 || waves_per_block > kWavesPerBlock) return hipErrorInvalidValue;
-  // kModeBeamPair: one query per block, served by two waves on one LDS area
-  const bool pair = mode == kModeBeamPair;
-  if (pair && waves_per_block != 1) return hipErrorInvalidValue;
+  // kModeBeamPair / kModeBeamCoop: one query per block, served by two /
+  // kCoopWaves waves on one LDS area
+  const bool pair = mode == kModeBeamPair, coop = mode == kModeBeamCoop;
+  if ((pair || coop) && waves_per_block != 1) return hipErrorInvalidValue;
   const size_t lds = (size_t)lds_wave_bytes * waves_per_block;
   if (lds > 64 * 1024) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(fn, dim3(blocks), dim3(kWave * (pair ? 2 : waves_per_block)), lds, stream, a,
-                     lds_wave_bytes);
+  const int threads = kWave * (coop ? kCoopWaves : (pair ? 2 : waves_per_block));
+  hipLaunchKernelGGL(fn, dim3(blocks), dim3(threads), lds, stream, a, lds_wave_bytes);
   return hipGetLastError();
 }
 

@@ -572,13 +572,21 @@ DevIndex dev_index(nmsamd_index* h) {
   return ix;
 }
 
+// Beam mode on the cooperative kernel (one query per block of kCoopWaves
+// waves): forced by setBeamWaves(8), and the only kernel for beam widths
+// above 16.
+bool use_coop(const nmsamd_index* h) { return h->beam > 0 && (h->beam_waves == 8 || h->beam > 16); }
+
 // Neighbour entries one search step holds in LDS: beam mode takes up to
-// kMaxStepEntries / maxM0 expansions per step, W * maxM0 entries rounded up to
-// whole rows of 64; the other modes expand one list of <= 64 entries.
+// kMaxStepEntries / maxM0 expansions per step (kMaxCoopEntries on the
+// cooperative kernel), W * maxM0 entries rounded up to whole rows of 64; the
+// other modes expand one list of <= 64 entries.
 constexpr int kMaxStepEntries = 512;
+constexpr int kMaxCoopEntries = 2048;
 int step_entries(const nmsamd_index* h) {
   if (h->beam <= 0) return 128;
-  const int w = std::max(1, std::min(h->beam, kMaxStepEntries / (int)h->maxM0));
+  const int maxe = use_coop(h) ? kMaxCoopEntries : kMaxStepEntries;
+  const int w = std::max(1, std::min(h->beam, maxe / (int)h->maxM0));
   return std::max(128, (w * (int)h->maxM0 + 63) / 64 * 64);
 }
 
@@ -675,8 +683,11 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
     cap = std::max(cap, std::max(max_ef, max_k) + r);
   }
   const int ent = beam ? step_entries(h) : 128;
-  const bool vbits = use_bit_marks(h, max_ef, beam);
-  const int lds = lds_wave_bytes(cap, h->dstride, ent, beam);
+  const bool coop = beam && use_coop(h);
+  // the cooperative kernel takes bit marks (its visited probe is an atomic OR)
+  // and a second items buffer
+  const bool vbits = coop || use_bit_marks(h, max_ef, beam);
+  const int lds = lds_wave_bytes(cap, h->dstride, ent, beam) + (coop ? 8 * ent : 0);
   if ((size_t)lds > kLdsPerCu)
     fail(NMSAMD_EINVAL, "efSearch / k too large: the search array of " + std::to_string(cap) +
                             " items exceeds the LDS of one wave");
@@ -701,20 +712,25 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   // automatically only where LDS already limits a CU to three queries.  The
   // limit is the hardware's, not only the grid's: a two-wave block reserves
   // more than a quarter of the CU's LDS, so a fourth block never fits.
+  // A cooperative block holds the CU's register file at two waves per SIMD:
+  // one block, one query per CU.
   const size_t fit = kLdsPerCu / lds;
-  const bool pair = beam && (h->beam_waves == 2 || (h->beam_waves == 0 && fit <= 3));
+  const bool pair = beam && !coop && (h->beam_waves == 2 || (h->beam_waves == 0 && fit <= 3));
   bool defer = false;
   const int lds_launch = pair ? pair_lds(lds, ent, &defer) : lds;
-  const int wpb = pair ? 1 : (int)std::max<size_t>(1, std::min<size_t>(kWavesPerBlock, fit));
-  const int per_cu = (int)std::max<size_t>(
+  const int wpb = (pair || coop) ? 1 : (int)std::max<size_t>(1, std::min<size_t>(kWavesPerBlock, fit));
+  const int per_cu = coop ? 1 : (int)std::max<size_t>(
       1, std::min<size_t>(pair ? 3 : 8, kLdsPerCu / ((size_t)lds_launch * wpb)));
   // the one-wave kernel: wide (every vector load of a round in flight, the
   // register file of one wave per SIMD) where the footprint already holds a
   // CU to four waves, or as forced by setBeamGather; forced wide, the grid
   // keeps to the four waves per CU it can hold
-  const bool wide = !pair && (h->beam_gather == 2 || (h->beam_gather == 0 && wpb * per_cu <= 4));
+  const bool wide = !pair && !coop &&
+                    (h->beam_gather == 2 || (h->beam_gather == 0 && wpb * per_cu <= 4));
   const int per_cu_run = wide ? std::max(1, std::min(per_cu, 4 / wpb)) : per_cu;
-  const int mode = !beam ? kModeAll : (pair ? kModeBeamPair : (wide ? kModeBeamWide : kModeBeam));
+  const int mode = !beam ? kModeAll
+                         : (coop ? kModeBeamCoop
+                                 : (pair ? kModeBeamPair : (wide ? kModeBeamWide : kModeBeam)));
   int64_t max_blocks = (int64_t)h->cu_count * per_cu_run;
   const int64_t heap_cap = any_old ? std::max<int64_t>(65536, 64 * (int64_t)max_ef) : 0;
   // a slot of the external scratch holds n + 1 epoch bytes, or with bit marks
@@ -1332,7 +1348,7 @@ int nmsamd_set_precision(nmsamd_index* h, int storage, int rerank_mult) {
 int nmsamd_set_beam(nmsamd_index* h, int beam) {
   return guarded([&] {
     if (!h) fail(NMSAMD_EINVAL, "null handle");
-    if (beam < 0 || beam > 16) fail(NMSAMD_EINVAL, "beam must be 0 (exact V1Merge) or 1..16");
+    if (beam < 0 || beam > 64) fail(NMSAMD_EINVAL, "beam must be 0 (exact V1Merge) or 1..64");
     h->beam = beam;
   });
 }
@@ -1358,7 +1374,8 @@ int nmsamd_set_visited_marks(nmsamd_index* h, int marks) {
 int nmsamd_set_beam_waves(nmsamd_index* h, int waves) {
   return guarded([&] {
     if (!h) fail(NMSAMD_EINVAL, "null handle");
-    if (waves < 0 || waves > 2) fail(NMSAMD_EINVAL, "beam waves must be 0 (by LDS footprint), 1 or 2");
+    if (waves < 0 || (waves > 2 && waves != 8))
+      fail(NMSAMD_EINVAL, "beam waves must be 0 (by LDS footprint), 1, 2 or 8 (cooperative)");
     h->beam_waves = waves;
   });
 }
@@ -1462,6 +1479,7 @@ struct nmsamd_server {
   int pair = 0;  // two waves per query (kModeBeamPair), from the LDS footprint
   int defer = 0;  // ... with the deferred rebuild (a second items buffer)
   int vbits = 0;  // visited marks as bits
+  int coop = 0;   // cooperative kernel (one query per block of kCoopWaves waves)
   DevIndex ix{};
   int slots_per_lane = 0;
   bool use_graph = false;
@@ -1513,7 +1531,8 @@ void lane_launch(nmsamd_server* s, ServerLane& L) {
   SearchArgs a = server_args(s, L);
   hip_check(launch_fill_i32(L.work.p, 0, 2, L.stream), "fill");
   // the server runs V1Merge, so beam batches always take the beam-only kernels
-  const int mode = a.beam > 0 ? (s->pair ? kModeBeamPair : (s->wide ? kModeBeamWide : kModeBeam))
+  const int mode = a.beam > 0 ? (s->coop ? kModeBeamCoop
+                                          : (s->pair ? kModeBeamPair : (s->wide ? kModeBeamWide : kModeBeam)))
                               : kModeAll;
   hip_check(launch_search(a, L.blocks, s->wpb, s->lds, mode, L.stream), "search kernel launch");
 }
@@ -1545,34 +1564,44 @@ int nmsamd_server_open(nmsamd_index* h, int lanes, int max_queries, int max_ef, 
       s->cap = std::max(s->cap, std::max(max_ef, max_k) +
                                     std::min(std::max(max_ef, max_k), h->rerank_mult * max_k));
     s->ent = step_entries(h);
-    s->lds = lds_wave_bytes(s->cap, h->dstride, s->ent, h->beam > 0);
+    s->coop = use_coop(h) ? 1 : 0;
+    s->lds = lds_wave_bytes(s->cap, h->dstride, s->ent, h->beam > 0) + (s->coop ? 8 * s->ent : 0);
     if ((size_t)s->lds > kLdsPerCu) fail(NMSAMD_EINVAL, "max_ef / max_k exceed the LDS of one wave");
     ensure_lowp(h);  // finished on return: the lanes' streams read the copy at once
     s->storage = h->storage;
     s->rerank = (h->storage != kF32 && h->rerank_mult > 0) ? h->rerank_mult : 0;
     s->beam = h->beam;
-    s->vbits = use_bit_marks(h, max_ef, h->beam > 0) ? 1 : 0;
+    s->vbits = (s->coop || use_bit_marks(h, max_ef, h->beam > 0)) ? 1 : 0;
     // the batch entry points' launch shape: two waves per query where the
     // footprint limits a CU to three queries (one query per block, LDS
     // padded so no fourth block fits), else one, wide where the footprint
     // holds a CU to four waves
+    // (the cooperative kernel: one block of kCoopWaves waves per CU)
     const size_t fit = kLdsPerCu / s->lds;
-    s->pair = (h->beam > 0 && h->beam_waves != 1 && (h->beam_waves == 2 || fit <= 3)) ? 1 : 0;
+    s->pair = (h->beam > 0 && !s->coop && h->beam_waves != 1 && (h->beam_waves == 2 || fit <= 3))
+                  ? 1 : 0;
     if (s->pair) {
       bool defer = false;
       s->lds = pair_lds(s->lds, s->ent, &defer);
       s->defer = defer ? 1 : 0;
     }
-    s->wpb = s->pair ? 1 : (int)std::max<size_t>(1, std::min<size_t>(kWavesPerBlock, fit));
-    const int per_cu = (int)std::max<size_t>(
+    s->wpb = (s->pair || s->coop) ? 1
+                                  : (int)std::max<size_t>(1, std::min<size_t>(kWavesPerBlock, fit));
+    const int per_cu = s->coop ? 1 : (int)std::max<size_t>(
         1, std::min<size_t>(s->pair ? 3 : 8, kLdsPerCu / ((size_t)s->lds * s->wpb)));
-    s->wide = (h->beam > 0 && !s->pair && s->wpb * per_cu <= 4) ? 1 : 0;
+    s->wide = (h->beam > 0 && !s->pair && !s->coop && s->wpb * per_cu <= 4) ? 1 : 0;
     // the device's resident waves at this footprint, shared by the lanes
     const int64_t total_blocks = (int64_t)h->cu_count * per_cu;
-    const int64_t blocks_per_lane = std::max<int64_t>(1, total_blocks / lanes);
+    // cooperative lanes each launch one block per CU: a batch spreads over the
+    // whole device, and concurrent lanes' blocks take CUs as earlier ones finish
+    const int64_t blocks_per_lane =
+        s->coop ? total_blocks : std::max<int64_t>(1, total_blocks / lanes);
     s->slots_per_lane = (int)(blocks_per_lane * s->wpb);
     const int64_t slots = (int64_t)s->slots_per_lane * lanes;
-    h->vstride_server = ((int64_t)h->n + 1 + 255) / 256 * 256;
+    // (bit marks of the cooperative kernel: the (n + 128) / 128 16-byte words a
+    // query clears; else n + 1 epoch bytes)
+    h->vstride_server = s->coop ? (((int64_t)h->n + 128) / 128 * 16 + 255) / 256 * 256
+                                : ((int64_t)h->n + 1 + 255) / 256 * 256;
     s->visited.ensure((size_t)slots * h->vstride_server);
     s->epoch.ensure((size_t)slots);
     hip_check(hipMemset(s->visited.p, 0, (size_t)slots * h->vstride_server), "memset");
@@ -1685,7 +1714,7 @@ int nmsamd_server_info(const nmsamd_server* s, int64_t* o) {
     if (!s || !o) fail(NMSAMD_EINVAL, "null argument");
     o[0] = (int64_t)s->lanes.size(); o[1] = s->max_queries; o[2] = s->max_ef; o[3] = s->max_k;
     o[4] = s->slots_per_lane; o[5] = s->wpb; o[6] = s->use_graph ? 1 : 0;
-    o[7] = s->pair ? 2 : s->wide;
+    o[7] = s->coop ? 3 : (s->pair ? 2 : s->wide);
   });
 }
 

@@ -60,12 +60,13 @@ def test_fullsize_exact_search_equals_oracle(big, ef):
     np.testing.assert_array_equal(counts, oc)
 
 
-@pytest.mark.parametrize("beam", [4, 16])
-@pytest.mark.parametrize("waves", [1, 2])
+@pytest.mark.parametrize("beam,waves", [(4, 1), (16, 1), (4, 2), (16, 2), (16, 8), (32, 8),
+                                        (64, 8)])
 def test_fullsize_beam_search_equals_model(big, beam, waves):
     # at the benchmark's efSearch the array spans 74 chunks of 64 and a step
-    # merges up to a few hundred items: the device equals the CPU model of the
-    # beam search row for row, with one wave per query and with two
+    # merges up to a few hundred items (beam 64: about a thousand): the device
+    # equals the CPU model of the beam search row for row, with one wave per
+    # query, two, and the cooperative block of eight
     path, idx, q, _, _ = big
     idx.setBeam(beam)
     idx.setBeamWaves(waves)

@@ -80,3 +80,32 @@ def test_fullsize_server_two_wave_lanes_at_the_recall_point(big):
         np.testing.assert_array_equal(gd, dists[:, :r.k], err_msg=f"request {r.rid}")
     idx.setPrecision("fp32", 0)
     idx.setBeam(0)
+
+
+@pytest.mark.parametrize("storage,rerank", [("fp32", 0), ("bf16", 2)])
+def test_fullsize_server_coop_lanes_at_the_recall_point(big, storage, rerank):
+    # every request at efSearch 4736 on the cooperative kernel (beam 64, a
+    # block of eight waves per query): the served rows equal the batch entry
+    # point's for the same requests
+    idx, pool = big
+    idx.setPrecision(storage, rerank)
+    idx.setBeam(64)
+    idx.setBeamWaves(8)
+    srv = serving.Server(idx, lanes=4, max_queries=256, max_ef=4736, max_k=10, use_graph=True)
+    assert srv.info()["beam_kernel"] == 3
+    reqs = serving.make_requests(len(pool), rate_qps=20_000, duration_s=0.3, seed=13,
+                                 size_range=(1, 64), ks=(10,), ef_range=(4736, 4736))
+    st = serving.run_poisson(srv, pool, reqs, keep_results=True)
+    srv.close()
+    print(f"{storage} coop 64, ef 4736, offered 20k q/s: {st}", flush=True)
+    for r in reqs[::max(1, len(reqs) // 12)]:
+        q = pool[r.rows]
+        ids, dists, counts = idx.knnQueryBatchUneven(q, np.full(len(q), r.k, np.int32),
+                                                     np.full(len(q), r.ef, np.int32))
+        gi, gd, gc = r.result
+        np.testing.assert_array_equal(gc, counts, err_msg=f"request {r.rid}")
+        np.testing.assert_array_equal(gi, ids[:, :r.k], err_msg=f"request {r.rid}")
+        np.testing.assert_array_equal(gd, dists[:, :r.k], err_msg=f"request {r.rid}")
+    idx.setPrecision("fp32", 0)
+    idx.setBeamWaves(0)
+    idx.setBeam(0)

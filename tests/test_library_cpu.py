@@ -82,16 +82,21 @@ def test_extension_setters_validate_their_arguments():
     # beam width, waves per beam query and precision: accepted ranges and the
     # ValueError of anything outside them (no device needed)
     idx = nmslib.init(space="l2")
-    for w in (0, 1, 16):
+    for w in (0, 1, 16, 64):
         idx.setBeam(w)
-    for bad in (-1, 17):
+    for bad in (-1, 65):
         with pytest.raises(ValueError, match="beam must be"):
             idx.setBeam(bad)
-    for n in (0, 1, 2):
+    for n in (0, 1, 2, 8):
         idx.setBeamWaves(n)
-    for bad in (-1, 3):
+    for bad in (-1, 3, 4):
         with pytest.raises(ValueError, match="beam waves must be"):
             idx.setBeamWaves(bad)
+    for v in (0, 1, 2):
+        idx.setVisitedMarks(v)
+    for bad in (-1, 3):
+        with pytest.raises(ValueError, match="visited marks must be"):
+            idx.setVisitedMarks(bad)
     idx.setPrecision("bf16", 2)
     idx.setPrecision("fp32", 0)
     with pytest.raises(ValueError):
