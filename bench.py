@@ -207,9 +207,12 @@ FORMS = {
     "fp8-beam16-rerank": ("fp8", 2, 16, 0),
     "fp32-coop16": ("fp32", 0, 16, 8),
     "fp32-coop32": ("fp32", 0, 32, 8),
+    "fp32-coop40": ("fp32", 0, 40, 8),
+    "fp32-coop48": ("fp32", 0, 48, 8),
     "fp32-coop64": ("fp32", 0, 64, 8),
     "bf16-coop16-rerank": ("bf16", 2, 16, 8),
     "bf16-coop32-rerank": ("bf16", 2, 32, 8),
+    "bf16-coop48-rerank": ("bf16", 2, 48, 8),
     "bf16-coop64-rerank": ("bf16", 2, 64, 8),
 }
 

@@ -60,6 +60,20 @@ def test_fullsize_exact_search_equals_oracle(big, ef):
     np.testing.assert_array_equal(counts, oc)
 
 
+def test_fullsize_exact_search_at_the_recall_point_equals_oracle(big):
+    # efSearch 4736, where recall@10 first reaches 0.95: the exact search's
+    # rows, distances and counts equal the oracle's on a sample of 48 queries
+    path, idx, q, _, _ = big
+    idx.setBeam(0)
+    idx.setPrecision("fp32", 0)
+    idx.setQueryTimeParams({"efSearch": 4736, "algoType": "v1merge"})
+    ids, dists, counts = idx.knnQueryBatchArrays(q[:48], 10)
+    oi, od, oc = O.OracleIndex(path).search(q[:48], 10, 4736, "v1merge", num_threads=8)
+    np.testing.assert_array_equal(ids, oi)
+    np.testing.assert_array_equal(dists.view(np.uint32), od.view(np.uint32))
+    np.testing.assert_array_equal(counts, oc)
+
+
 @pytest.mark.parametrize("beam,waves", [(4, 1), (16, 1), (4, 2), (16, 2), (16, 8), (32, 8),
                                         (64, 8)])
 def test_fullsize_beam_search_equals_model(big, beam, waves):

@@ -1,7 +1,8 @@
 """The no-allocation device entry point (nmsamd_knn_query_batch_device_noalloc,
 no reference counterpart) against nmsamd_knn_query_batch_device through the C
 ABI: the same ids, distances and counts, on two calls in a row whose second
-is smaller, for exact V1Merge, SearchOld and beam mode."""
+is smaller, for exact V1Merge, SearchOld, beam mode and the cooperative beam
+kernel (bit marks: slots of n / 8 bytes)."""
 import numpy as np
 import pytest
 
@@ -24,16 +25,18 @@ def _run(idx, entry, q, k):
     return ids.cpu().numpy(), dists.cpu().numpy(), counts.cpu().numpy()
 
 
-@pytest.mark.parametrize("name,params,beam,k", [
-    ("u128_l2", {"efSearch": 100, "algoType": "v1merge"}, 0, 10),
-    ("u10k_l2", {"efSearch": 64, "algoType": "old"}, 0, 10),
-    ("dec100_cos", {"efSearch": 300, "algoType": "v1merge"}, 8, 100),
+@pytest.mark.parametrize("name,params,beam,waves,k", [
+    ("u128_l2", {"efSearch": 100, "algoType": "v1merge"}, 0, 0, 10),
+    ("u10k_l2", {"efSearch": 64, "algoType": "old"}, 0, 0, 10),
+    ("dec100_cos", {"efSearch": 300, "algoType": "v1merge"}, 8, 0, 100),
+    ("u10k_l2", {"efSearch": 300, "algoType": "v1merge"}, 64, 8, 10),
 ])
-def test_no_alloc_entry_equals_device_entry(name, params, beam, k):
+def test_no_alloc_entry_equals_device_entry(name, params, beam, waves, k):
     m = F.META[name]
     idx = load(name, F.index_path(name), m["space"])
     idx.setQueryTimeParams(params)
     idx.setBeam(beam)
+    idx.setBeamWaves(waves)
     q = F.results(name)["queries"]
     idx.knnQueryBatchArrays(q[:4], k)  # the index reaches the device
     for rows in (slice(0, len(q)), slice(3, 3 + len(q) // 3)):  # the second call is smaller
@@ -43,4 +46,5 @@ def test_no_alloc_entry_equals_device_entry(name, params, beam, k):
         for g, r in zip(got, ref):
             np.testing.assert_array_equal(g, r)
         assert np.all(got[2] == min(k, m["n"]))
+    idx.setBeamWaves(0)
     idx.setBeam(0)
