@@ -425,8 +425,8 @@ def main():
     ap.add_argument("--ef", type=int, default=0, help="fixed efSearch (0: smallest of the ladder "
                     "reaching recall@10 >= 0.95)")
     ap.add_argument("--algo", default="v1merge")
-    ap.add_argument("--forms", default="fp32-exact,fp32-beam16,bf16-beam16-rerank,fp32-coop64,"
-                                       "bf16-coop64-rerank",
+    ap.add_argument("--forms", default="fp32-exact,fp32-beam16,bf16-beam16-rerank,fp32-coop48,"
+                                       "bf16-coop48-rerank",
                     help=f"comma list of {sorted(FORMS)}; the headline is the fastest")
     ap.add_argument("--nq", type=int, default=10000)
     ap.add_argument("--no-sweep", dest="sweep", action="store_false",

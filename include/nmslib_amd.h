@@ -185,8 +185,11 @@ int nmsamd_set_beam_waves(nmsamd_index* h, int waves);
  * (two waves per SIMD; one FP32 / two BF16 / four FP8 vectors per lane pair
  * and round), 2 = the wide one (one wave per SIMD; two FP32 / five BF16 / eight
  * FP8 vectors per lane pair, at most four waves per CU), 0 (default) = wide
- * where the LDS footprint already holds a CU to four waves.  Results do not
- * depend on it.  No reference counterpart (an autotuner knob). */
+ * where the LDS footprint already holds a CU to four waves.  On the
+ * cooperative kernel: 2 = one block per CU on the wide path, else two blocks
+ * per CU (one FP32 / two BF16 / four FP8 vectors per lane pair) where LDS
+ * holds two queries.  Results do not depend on it.  No reference counterpart
+ * (an autotuner knob). */
 int nmsamd_set_beam_gather(nmsamd_index* h, int gather);
 
 /* Visited marks of the beam-mode searches (the reference keeps a byte per
@@ -282,7 +285,8 @@ int nmsamd_server_poll(nmsamd_server* s, int lane);
  * and frees the lane. */
 int nmsamd_server_fetch(nmsamd_server* s, int lane, int32_t* ids, float* dists, int32_t* counts);
 /* lanes, max_queries, max_ef, max_k, slots per lane, waves per block, graph,
- * beam kernel (0 one wave, 1 one wave wide, 2 two waves, 3 cooperative) */
+ * beam kernel (0 one wave, 1 one wave wide, 2 two waves, 3 cooperative, 4 cooperative at two
+ * blocks per CU) */
 int nmsamd_server_info(const nmsamd_server* s, int64_t* out8);
 int nmsamd_server_close(nmsamd_server* s);
 

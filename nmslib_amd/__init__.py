@@ -258,7 +258,9 @@ class FloatIndex:
     def setBeamGather(self, gather: int = 0):
         """Vector loads in flight in the one-wave beam kernels: 1 = plain (two
         waves per SIMD), 2 = wide (one wave per SIMD, every load of a round in
-        flight), 0 = chosen from the LDS footprint.  Results do not depend on it."""
+        flight), 0 = chosen from the LDS footprint.  On the cooperative kernel,
+        2 keeps one block per CU on the wide path; otherwise two blocks share a
+        CU where LDS holds two queries.  Results do not depend on it."""
         _check(load_library().nmsamd_set_beam_gather(self._h, int(gather)))
 
     def setVisitedMarks(self, marks: int = 0):
@@ -290,7 +292,7 @@ class FloatIndex:
         if mode == "v1merge/old":
             keys = ("pick_adjacency", "visited", "distances", "admit_predict", "sort",
                     "rank_search", "merge", "advance")
-        elif mode == "beam (cooperative)":
+        elif mode.startswith("beam (cooperative"):
             keys = ("pick", "rows_marks", "distances_admit", "rank_new", "rank_array",
                     "records", "rebuild", "advance")
         else:
@@ -304,10 +306,11 @@ class FloatIndex:
         _check(load_library().nmsamd_last_launch(self._h, _ptr(out)))
         mode = int(out[0])
         kernel = {2: "hnsw_search_kernel_wide", 3: "hnsw_search_kernel_pair",
-                  4: "hnsw_search_kernel_coop"}.get(mode, "hnsw_search_kernel")
+                  4: "hnsw_search_kernel_coop", 5: "hnsw_search_kernel_coop2"}.get(
+                      mode, "hnsw_search_kernel")
         return {"kernel": kernel,
                 "mode": ("v1merge/old", "beam", "beam (wide)", "beam (two waves)",
-                         "beam (cooperative)")[mode],
+                         "beam (cooperative)", "beam (cooperative, two per CU)")[mode],
                 "blocks": int(out[1]),
                 "queries_per_block": int(out[2]), "lds_bytes_per_query": int(out[3]),
                 "entries_per_step": int(out[4])}

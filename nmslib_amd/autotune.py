@@ -7,8 +7,9 @@ except storage and rerank, whose recall is measured):
   1..64 = beam mode;
 * waves per query in beam mode: 1, 2 (the second wave computes each step's
   distances while the first rebuilds the sorted array) or 8 (the
-  cooperative kernel: one query per CU, every phase of a step shared by eight
-  waves, the only kernel for widths above 16);
+  cooperative kernel: a block of eight waves per query, every phase of a step
+  shared, two blocks per CU where LDS holds two queries; the only kernel for
+  widths above 16);
 * gather width of the one-wave kernels: plain (two waves per SIMD, one FP32 /
   two BF16 / four FP8 vectors per lane pair in flight) or wide (one wave per
   SIMD, two / five / eight);
@@ -44,7 +45,9 @@ STAGE1 = [
     ("bf16-beam16-rerank2", "bf16", 2, 16, 0, 0),
     ("fp8-beam8-rerank2", "fp8", 2, 8, 0, 0),
     ("fp32-coop32", "fp32", 0, 32, 8, 0),
+    ("fp32-coop48", "fp32", 0, 48, 8, 0),
     ("bf16-coop32-rerank2", "bf16", 2, 32, 8, 0),
+    ("bf16-coop48-rerank2", "bf16", 2, 48, 8, 0),
     ("bf16-coop64-rerank2", "bf16", 2, 64, 8, 0),
 ]
 # kept for callers that pass the old five forms explicitly
@@ -83,6 +86,8 @@ def _variants(form):
         out.append((f"{name}-1wave-wide", storage, rerank, beam, 1, 2))
         out.append((f"{name}-2waves", storage, rerank, beam, 2, 0))
         out.append((f"{name}-coop", storage, rerank, beam, 8, 0))
+    if waves == 8 or beam > 16:  # one cooperative block per CU, wide distance path
+        out.append((f"{name}-coop-wide", storage, rerank, beam, 8, 2))
     if storage != "fp32":
         for r in (1, 4):
             out.append((f"{name.rsplit('-rerank', 1)[0]}-rerank{r}", storage, r, beam, waves, 0))

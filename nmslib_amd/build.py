@@ -16,7 +16,7 @@ OUT = os.path.join(HERE, "libnmslib_amd.so")
 # (source, object name, extra defines): search_inst.hip is compiled once per
 # (vector storage, kernel mode) slice so the slices build in parallel
 SOURCES = ([("search_inst.hip", f"search_st{st}_m{m}", [f"-DNMS_ST={st}", f"-DNMS_MODE={m}"])
-            for st in range(3) for m in range(5)]
+            for st in range(3) for m in range(6)]
            + [(f, f, []) for f in ("launch.hip", "lowp_convert.hip", "bruteforce.hip",
                                    "bruteforce_exact.hip", "nmslib_amd.cpp", "hnsw_build.cpp")])
 # host-only sources: the index builder (x86 SSE, no device code)

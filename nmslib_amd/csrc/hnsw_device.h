@@ -121,7 +121,8 @@ enum AlgoMode : int { kAlgoHybrid = 0, kAlgoOld = 1, kAlgoV1Merge = 2 };
 // LDS lacks room for a second items buffer, each wave rebuilds half of it).
 enum KernelMode : int {
   kModeAll = 0, kModeBeam = 1, kModeBeamWide = 2, kModeBeamPair = 3,
-  kModeBeamCoop = 4  // one query per block of kCoopWaves waves (beam up to 64)
+  kModeBeamCoop = 4,  // one query per block of kCoopWaves waves (beam up to 64)
+  kModeBeamCoop2 = 5  // the same, two blocks per CU (registers of four waves per SIMD)
 };
 constexpr int kCoopWaves = 8;
 

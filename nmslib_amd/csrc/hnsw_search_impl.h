@@ -2278,7 +2278,7 @@ __device__ __forceinline__ void coop_sort_new(uint2* items, uint2* X, int m, int
   __syncthreads();
 }
 
-template <int D, int FUNC, int ST>
+template <int D, int FUNC, int ST, int DPATH>
 __device__ __forceinline__ void coop_query(const SearchArgs& a, const WaveLds& w, const Marks& vis,
                                            int qi, int k, int ef, int wave, int lane, int h,
                                            uint64_t ltm) {
@@ -2424,8 +2424,8 @@ __device__ __forceinline__ void coop_query(const SearchArgs& a, const WaveLds& w
     // 3. distances of a slice of the candidates per wave, and the admission
     const int per = (ncand + kCoopWaves - 1) / kCoopWaves;
     const int s0 = min(ncand, wave * per), s1 = min(ncand, s0 + per);
-    dist_cands<D, FUNC, ST, 1>(w.qs, ix, w.cand + s0, w.cdist + s0, s1 - s0, lane, h, fallback,
-                               a.err);
+    dist_cands<D, FUNC, ST, DPATH>(w.qs, ix, w.cand + s0, w.cdist + s0, s1 - s0, lane, h, fallback,
+                                   a.err);
     WSYNC();
     for (int b = s0; b < s1; b += 64) {
       const int s = b + lane;
@@ -2546,7 +2546,7 @@ __device__ __forceinline__ void coop_query(const SearchArgs& a, const WaveLds& w
 }
 
 // One query per block of kCoopWaves waves (coop_query).
-template <int D, int FUNC, int ST>
+template <int D, int FUNC, int ST, int DPATH>
 __device__ __forceinline__ void coop_body(const SearchArgs& a) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int NT = kCoopWaves * kWave;
@@ -2604,7 +2604,7 @@ __device__ __forceinline__ void coop_body(const SearchArgs& a) {
       __syncthreads();
       continue;
     }
-    coop_query<D, FUNC, ST>(a, w, vis, qi, k, ef, wave, lane, h, ltm);
+    coop_query<D, FUNC, ST, DPATH>(a, w, vis, qi, k, ef, wave, lane, h, ltm);
     __syncthreads();
     if (a.q_ticks && threadIdx.x == 0)
       a.q_ticks[qi] = (uint32_t)(__builtin_amdgcn_s_memrealtime() - t_start);
@@ -2888,10 +2888,20 @@ __global__ void __launch_bounds__(2 * kWave) __attribute__((amdgpu_waves_per_eu(
 }
 
 // kModeBeamCoop: one query per block of kCoopWaves waves, two waves per SIMD
+// (one block per CU), the wide distance path
 template <int D, int FUNC, int ST>
 __global__ void __launch_bounds__(kCoopWaves * kWave) __attribute__((amdgpu_waves_per_eu(2)))
     hnsw_search_kernel_coop(SearchArgs a, int) {
-  coop_body<D, FUNC, ST>(a);
+  coop_body<D, FUNC, ST, 1>(a);
+}
+
+// kModeBeamCoop2: the same with the register file of four waves per SIMD, so
+// two blocks share a CU where LDS holds two queries; one FP32 vector (two BF16,
+// four FP8) per lane pair in flight
+template <int D, int FUNC, int ST>
+__global__ void __launch_bounds__(kCoopWaves * kWave) __attribute__((amdgpu_waves_per_eu(4)))
+    hnsw_search_kernel_coop2(SearchArgs a, int) {
+  coop_body<D, FUNC, ST, ST == kF32 ? 0 : 2>(a);
 }
 
 using KernelFn = void (*)(SearchArgs, int);
@@ -2901,6 +2911,7 @@ KernelFn kernel_of() {
   if constexpr (MODE == kModeBeamWide) return hnsw_search_kernel_wide<D, FUNC, ST>;
   else if constexpr (MODE == kModeBeamPair) return hnsw_search_kernel_pair<D, FUNC, ST>;
   else if constexpr (MODE == kModeBeamCoop) return hnsw_search_kernel_coop<D, FUNC, ST>;
+  else if constexpr (MODE == kModeBeamCoop2) return hnsw_search_kernel_coop2<D, FUNC, ST>;
   else return hnsw_search_kernel<D, FUNC, ST, MODE>;
 }
 

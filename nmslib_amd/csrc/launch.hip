@@ -15,7 +15,8 @@ using KernelFn = void (*)(SearchArgs, int);
   KernelFn pick_kernel_st##S##_m1(int dim, int func);  \
   KernelFn pick_kernel_st##S##_m2(int dim, int func);  \
   KernelFn pick_kernel_st##S##_m3(int dim, int func);  \
-  KernelFn pick_kernel_st##S##_m4(int dim, int func);
+  KernelFn pick_kernel_st##S##_m4(int dim, int func);  \
+  KernelFn pick_kernel_st##S##_m5(int dim, int func);
 NMS_DECL(0)
 NMS_DECL(1)
 NMS_DECL(2)
@@ -23,14 +24,14 @@ NMS_DECL(2)
 
 static KernelFn pick_kernel(int storage, int dim, int func, int mode) {
   using Pick = KernelFn (*)(int, int);
-  static const Pick table[3][5] = {
+  static const Pick table[3][6] = {
       {pick_kernel_st0_m0, pick_kernel_st0_m1, pick_kernel_st0_m2, pick_kernel_st0_m3,
-       pick_kernel_st0_m4},
+       pick_kernel_st0_m4, pick_kernel_st0_m5},
       {pick_kernel_st1_m0, pick_kernel_st1_m1, pick_kernel_st1_m2, pick_kernel_st1_m3,
-       pick_kernel_st1_m4},
+       pick_kernel_st1_m4, pick_kernel_st1_m5},
       {pick_kernel_st2_m0, pick_kernel_st2_m1, pick_kernel_st2_m2, pick_kernel_st2_m3,
-       pick_kernel_st2_m4}};
-  if (storage < 0 || storage > 2 || mode < 0 || mode > 4) return nullptr;
+       pick_kernel_st2_m4, pick_kernel_st2_m5}};
+  if (storage < 0 || storage > 2 || mode < 0 || mode > 5) return nullptr;
   return table[storage][mode](dim, func);
 }
 
@@ -41,7 +42,7 @@ hipError_t launch_search(const SearchArgs& a, int blocks, int waves_per_block, i
   if (waves_per_block < 1 || waves_per_block > kWavesPerBlock) return hipErrorInvalidValue;
   // kModeBeamPair / kModeBeamCoop: one query per block, served by two /
   // kCoopWaves waves on one LDS area
-  const bool pair = mode == kModeBeamPair, coop = mode == kModeBeamCoop;
+  const bool pair = mode == kModeBeamPair, coop = mode == kModeBeamCoop || mode == kModeBeamCoop2;
   if ((pair || coop) && waves_per_block != 1) return hipErrorInvalidValue;
   const size_t lds = (size_t)lds_wave_bytes * waves_per_block;
   if (lds > 64 * 1024) {

@@ -576,6 +576,9 @@ DevIndex dev_index(nmsamd_index* h) {
 // waves): forced by setBeamWaves(8), and the only kernel for beam widths
 // above 16.
 bool use_coop(const nmsamd_index* h) { return h->beam > 0 && (h->beam_waves == 8 || h->beam > 16); }
+// Two cooperative blocks per CU (kModeBeamCoop2) where LDS holds two queries,
+// unless setBeamGather(2) asks for the wide distance path of one per CU.
+bool coop_two_per_cu(const nmsamd_index* h, size_t fit) { return fit >= 2 && h->beam_gather != 2; }
 
 // Neighbour entries one search step holds in LDS: beam mode takes up to
 // kMaxStepEntries / maxM0 expansions per step (kMaxCoopEntries on the
@@ -712,14 +715,16 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   // automatically only where LDS already limits a CU to three queries.  The
   // limit is the hardware's, not only the grid's: a two-wave block reserves
   // more than a quarter of the CU's LDS, so a fourth block never fits.
-  // A cooperative block holds the CU's register file at two waves per SIMD:
-  // one block, one query per CU.
+  // A cooperative block takes the CU's register file at two waves per SIMD
+  // (one query per CU), or half of it where LDS holds two queries
+  // (kModeBeamCoop2).
   const size_t fit = kLdsPerCu / lds;
+  const bool coop2 = coop && coop_two_per_cu(h, fit);
   const bool pair = beam && !coop && (h->beam_waves == 2 || (h->beam_waves == 0 && fit <= 3));
   bool defer = false;
   const int lds_launch = pair ? pair_lds(lds, ent, &defer) : lds;
   const int wpb = (pair || coop) ? 1 : (int)std::max<size_t>(1, std::min<size_t>(kWavesPerBlock, fit));
-  const int per_cu = coop ? 1 : (int)std::max<size_t>(
+  const int per_cu = coop ? (coop2 ? 2 : 1) : (int)std::max<size_t>(
       1, std::min<size_t>(pair ? 3 : 8, kLdsPerCu / ((size_t)lds_launch * wpb)));
   // the one-wave kernel: wide (every vector load of a round in flight, the
   // register file of one wave per SIMD) where the footprint already holds a
@@ -729,7 +734,7 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
                     (h->beam_gather == 2 || (h->beam_gather == 0 && wpb * per_cu <= 4));
   const int per_cu_run = wide ? std::max(1, std::min(per_cu, 4 / wpb)) : per_cu;
   const int mode = !beam ? kModeAll
-                         : (coop ? kModeBeamCoop
+                         : (coop ? (coop2 ? kModeBeamCoop2 : kModeBeamCoop)
                                  : (pair ? kModeBeamPair : (wide ? kModeBeamWide : kModeBeam)));
   int64_t max_blocks = (int64_t)h->cu_count * per_cu_run;
   const int64_t heap_cap = any_old ? std::max<int64_t>(65536, 64 * (int64_t)max_ef) : 0;
@@ -1480,6 +1485,7 @@ struct nmsamd_server {
   int defer = 0;  // ... with the deferred rebuild (a second items buffer)
   int vbits = 0;  // visited marks as bits
   int coop = 0;   // cooperative kernel (one query per block of kCoopWaves waves)
+  int coop2 = 0;  // two cooperative blocks per CU
   DevIndex ix{};
   int slots_per_lane = 0;
   bool use_graph = false;
@@ -1531,7 +1537,7 @@ void lane_launch(nmsamd_server* s, ServerLane& L) {
   SearchArgs a = server_args(s, L);
   hip_check(launch_fill_i32(L.work.p, 0, 2, L.stream), "fill");
   // the server runs V1Merge, so beam batches always take the beam-only kernels
-  const int mode = a.beam > 0 ? (s->coop ? kModeBeamCoop
+  const int mode = a.beam > 0 ? (s->coop ? (s->coop2 ? kModeBeamCoop2 : kModeBeamCoop)
                                           : (s->pair ? kModeBeamPair : (s->wide ? kModeBeamWide : kModeBeam)))
                               : kModeAll;
   hip_check(launch_search(a, L.blocks, s->wpb, s->lds, mode, L.stream), "search kernel launch");
@@ -1587,7 +1593,8 @@ int nmsamd_server_open(nmsamd_index* h, int lanes, int max_queries, int max_ef, 
     }
     s->wpb = (s->pair || s->coop) ? 1
                                   : (int)std::max<size_t>(1, std::min<size_t>(kWavesPerBlock, fit));
-    const int per_cu = s->coop ? 1 : (int)std::max<size_t>(
+    s->coop2 = (s->coop && coop_two_per_cu(h, fit)) ? 1 : 0;
+    const int per_cu = s->coop ? (s->coop2 ? 2 : 1) : (int)std::max<size_t>(
         1, std::min<size_t>(s->pair ? 3 : 8, kLdsPerCu / ((size_t)s->lds * s->wpb)));
     s->wide = (h->beam > 0 && !s->pair && !s->coop && s->wpb * per_cu <= 4) ? 1 : 0;
     // the device's resident waves at this footprint, shared by the lanes
@@ -1714,7 +1721,7 @@ int nmsamd_server_info(const nmsamd_server* s, int64_t* o) {
     if (!s || !o) fail(NMSAMD_EINVAL, "null argument");
     o[0] = (int64_t)s->lanes.size(); o[1] = s->max_queries; o[2] = s->max_ef; o[3] = s->max_k;
     o[4] = s->slots_per_lane; o[5] = s->wpb; o[6] = s->use_graph ? 1 : 0;
-    o[7] = s->coop ? 3 : (s->pair ? 2 : s->wide);
+    o[7] = s->coop ? (s->coop2 ? 4 : 3) : (s->pair ? 2 : s->wide);
   });
 }
 

@@ -1,8 +1,9 @@
 // One slice of the search kernel instantiations: vector storage NMS_ST
 // (0 FP32, 1 BF16, 2 FP8) x kernel mode NMS_MODE (0 every mode, 1 beam,
-// 2 beam wide, 3 beam two waves, 4 beam cooperative), over the specialised dimensions {96, 100,
+// 2 beam wide, 3 beam two waves, 4 beam cooperative, 5 beam cooperative at two
+// blocks per CU), over the specialised dimensions {96, 100,
 // 128, any} and the distance functions.  The build compiles this file once per
-// slice so the fifteen slices compile in parallel; launch.hip picks the slice.
+// slice so the eighteen slices compile in parallel; launch.hip picks the slice.
 #include "hnsw_search_impl.h"
 
 #if !defined(NMS_ST) || !defined(NMS_MODE)

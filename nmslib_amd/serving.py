@@ -66,7 +66,8 @@ class Server:
         out = np.zeros(8, np.int64)
         _check(_lib().nmsamd_server_info(self._h, _ptr(out)))
         # beam_kernel: 0 = one wave per query (plain), 1 = one wave (wide),
-        # 2 = two waves per query, 3 = cooperative (a block of eight waves)
+        # 2 = two waves per query, 3 = cooperative (a block of eight waves),
+        # 4 = cooperative at two blocks per CU
         keys = ("lanes", "max_queries", "max_ef", "max_k", "slots_per_lane", "waves_per_block",
                 "graph", "beam_kernel")
         return {k: int(v) for k, v in zip(keys, out)}

@@ -237,11 +237,14 @@ def test_gpu_beam_mode_equals_model(name, beam, waves):
 @pytest.mark.parametrize("name", ["u128_l2", "hn128_l2", "dec100_cos", "gmm96_nd", "u10k_l2",
                                   "u67_l1", "u64_linf"])
 @pytest.mark.parametrize("beam", [1, 2, 5, 16, 32, 64])
-def test_gpu_coop_beam_equals_model(name, beam):
+@pytest.mark.parametrize("gather", [0, 2])
+def test_gpu_coop_beam_equals_model(name, beam, gather):
     # the cooperative kernel (one query per block of eight waves; bit marks set
-    # by atomic ORs that also dedupe; ranks by counting; the rebuild in rounds
-    # of 64 chunks) against the CPU model: ids, distances and work counters
-    # identical, up to 2048 entries per step
+    # by atomic ORs that also dedupe; per-wave sorted runs ranked by binary
+    # search; the rebuild in rounds of 64 chunks) against the CPU model: ids,
+    # distances and work counters identical, up to 2048 entries per step.
+    # gather 0: two blocks per CU where LDS holds two queries (always, at these
+    # sizes); gather 2: one block per CU with the wide distance path
     m = F.META[name]
     idx = load(name, F.index_path(name), m["space"])
     oracle = O.OracleIndex(F.index_path(name))
@@ -249,20 +252,24 @@ def test_gpu_coop_beam_equals_model(name, beam):
     q = F.results(name)["queries"]
     idx.setBeam(beam)
     idx.setBeamWaves(8)
+    idx.setBeamGather(gather)
     idx.collectStats(True)
     for k, ef in [(10, 64), (100, 32), (10, 300), (1, 700)]:
         idx.setQueryTimeParams({"efSearch": ef, "algoType": "v1merge"})
         ids, dists, counts = idx.knnQueryBatchArrays(q, k)
         st = idx.lastStats()
-        assert idx.lastLaunch()["kernel"] == "hnsw_search_kernel_coop"
+        launch = idx.lastLaunch()
+        two = gather == 0 and 163840 // launch["lds_bytes_per_query"] >= 2
+        assert launch["kernel"] == ("hnsw_search_kernel_coop2" if two else "hnsw_search_kernel_coop")
         rids, rdists, rcounts, rst = oracle.search_beam(q, k, ef, w, num_threads=8, with_stats=True)
-        msg = f"{name} beam={beam} coop k={k} ef={ef}"
+        msg = f"{name} beam={beam} {launch['kernel']} k={k} ef={ef}"
         print(msg, flush=True)
         np.testing.assert_array_equal(counts, rcounts, err_msg=msg)
         np.testing.assert_array_equal(ids, rids, err_msg=msg)
         np.testing.assert_array_equal(dists.view(np.uint32), rdists.view(np.uint32), err_msg=msg)
         for key in ("expansions", "dist_level0", "dist_upper", "list_ints_read"):
             assert st[key] == rst[key], (msg, key, st[key], rst[key])
+    idx.setBeamGather(0)
     idx.setBeamWaves(0)
     idx.setBeam(0)
 
