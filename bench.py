@@ -139,10 +139,11 @@ def serve_bench(idx, q, forms, rates, duration_s, lanes=8, max_queries=1024, ef_
     from nmslib_amd import serving
 
     out = []
-    for name, storage, rerank, beam, waves in forms:
+    for name, storage, rerank, beam, waves, gather in forms:
         idx.setPrecision(storage, rerank)
         idx.setBeam(beam)
         idx.setBeamWaves(waves)
+        idx.setBeamGather(gather)
         idx.setQueryTimeParams({"algoType": "v1merge"})
         srv = serving.Server(idx, lanes=lanes, max_queries=max_queries, max_ef=max_ef,
                              max_k=max(ks), use_graph=True)
@@ -168,6 +169,7 @@ def serve_bench(idx, q, forms, rates, duration_s, lanes=8, max_queries=1024, ef_
     idx.setPrecision("fp32", 0)
     idx.setBeam(0)
     idx.setBeamWaves(0)
+    idx.setBeamGather(0)
     return out
 
 
@@ -176,10 +178,11 @@ def single_query_latency(idx, q, forms, ef, n=32):
     the floor of any served request's latency at that operating point."""
     out = {}
     idx.setQueryTimeParams({"efSearch": ef, "algoType": "v1merge"})
-    for name, storage, rerank, beam, waves in forms:
+    for name, storage, rerank, beam, waves, gather in forms:
         idx.setPrecision(storage, rerank)
         idx.setBeam(beam)
         idx.setBeamWaves(waves)
+        idx.setBeamGather(gather)
         idx.knnQueryBatchArrays(q[:1], 10)
         ms = []
         for i in range(n):
@@ -192,6 +195,7 @@ def single_query_latency(idx, q, forms, ef, n=32):
     idx.setPrecision("fp32", 0)
     idx.setBeam(0)
     idx.setBeamWaves(0)
+    idx.setBeamGather(0)
     return out
 
 
@@ -425,8 +429,8 @@ def main():
     ap.add_argument("--ef", type=int, default=0, help="fixed efSearch (0: smallest of the ladder "
                     "reaching recall@10 >= 0.95)")
     ap.add_argument("--algo", default="v1merge")
-    ap.add_argument("--forms", default="fp32-exact,fp32-beam16,bf16-beam16-rerank,fp32-coop48,"
-                                       "bf16-coop48-rerank",
+    ap.add_argument("--forms", default="fp32-exact,fp32-beam16,bf16-beam16-rerank,fp32-coop64,"
+                                       "bf16-coop64-rerank",
                     help=f"comma list of {sorted(FORMS)}; the headline is the fastest")
     ap.add_argument("--nq", type=int, default=10000)
     ap.add_argument("--no-sweep", dest="sweep", action="store_false",
@@ -550,12 +554,14 @@ def main():
     serve = None
     if args.serve or args.serve_hi:
         rates = [float(r) for r in args.serve.split(",") if r]
-        # (name, storage, rerank multiple, beam width, waves per query)
-        serve_forms = [("fp32 exact V1Merge", "fp32", 0, 0, 0),
-                       ("fp32 beam16", "fp32", 0, 16, 0),
-                       ("bf16 beam16 + fp32 rerank x2", "bf16", 2, 16, 0),
-                       ("fp32 coop64", "fp32", 0, 64, 8),
-                       ("bf16 coop64 + fp32 rerank x2", "bf16", 2, 64, 8)]
+        # (name, storage, rerank multiple, beam width, waves per query, gather:
+        # 2 keeps the cooperative kernel at one block per CU)
+        serve_forms = [("fp32 exact V1Merge", "fp32", 0, 0, 0, 0),
+                       ("fp32 beam16", "fp32", 0, 16, 0, 0),
+                       ("bf16 beam16 + fp32 rerank x2", "bf16", 2, 16, 0, 0),
+                       ("fp32 coop64", "fp32", 0, 64, 8, 0),
+                       ("bf16 coop64 + fp32 rerank x2", "bf16", 2, 64, 8, 0),
+                       ("bf16 coop64 one block per CU + fp32 rerank x2", "bf16", 2, 64, 8, 2)]
         if args.serve_forms:
             serve_forms = [f for f in serve_forms if f[0] in args.serve_forms.split(",")]
         mix = serve_bench(idx, q, serve_forms, rates, args.serve_seconds, args.serve_lanes,

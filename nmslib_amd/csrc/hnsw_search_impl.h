@@ -2292,8 +2292,11 @@ __device__ __forceinline__ void coop_query(const SearchArgs& a, const WaveLds& w
   int* ctl = w.ctl;
   int* picks = reinterpret_cast<int*>(w.pfa);  // [W <= 64] (pfa, pfq, pfpos: 104 words)
   int* wcnt = picks + 64;                      // [kCoopWaves] per-wave counts of a pick round
-  int* lrank = w.cand;                         // after the distances: ranks in A of the new items
-  uint2* X = w.items2;                         // the step's new items, sorted
+  // after the admission the candidates' region holds the step's new items,
+  // sorted (X), and the admitted items' region, once sorted from, their ranks
+  // in A (lrank)
+  uint2* X = reinterpret_cast<uint2*>(w.cand);
+  int* lrank = reinterpret_cast<int*>(w.items);
   uint32_t* vw = reinterpret_cast<uint32_t*>(vis.p);
   unsigned long long nexp = 0, nd0 = 0, ndu = 1, nl = 0;
 

@@ -688,9 +688,8 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   const int ent = beam ? step_entries(h) : 128;
   const bool coop = beam && use_coop(h);
   // the cooperative kernel takes bit marks (its visited probe is an atomic OR)
-  // and a second items buffer
   const bool vbits = coop || use_bit_marks(h, max_ef, beam);
-  const int lds = lds_wave_bytes(cap, h->dstride, ent, beam) + (coop ? 8 * ent : 0);
+  const int lds = lds_wave_bytes(cap, h->dstride, ent, beam);
   if ((size_t)lds > kLdsPerCu)
     fail(NMSAMD_EINVAL, "efSearch / k too large: the search array of " + std::to_string(cap) +
                             " items exceeds the LDS of one wave");
@@ -1571,7 +1570,7 @@ int nmsamd_server_open(nmsamd_index* h, int lanes, int max_queries, int max_ef, 
                                     std::min(std::max(max_ef, max_k), h->rerank_mult * max_k));
     s->ent = step_entries(h);
     s->coop = use_coop(h) ? 1 : 0;
-    s->lds = lds_wave_bytes(s->cap, h->dstride, s->ent, h->beam > 0) + (s->coop ? 8 * s->ent : 0);
+    s->lds = lds_wave_bytes(s->cap, h->dstride, s->ent, h->beam > 0);
     if ((size_t)s->lds > kLdsPerCu) fail(NMSAMD_EINVAL, "max_ef / max_k exceed the LDS of one wave");
     ensure_lowp(h);  // finished on return: the lanes' streams read the copy at once
     s->storage = h->storage;
