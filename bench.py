@@ -139,11 +139,12 @@ def serve_bench(idx, q, forms, rates, duration_s, lanes=8, max_queries=1024, ef_
     from nmslib_amd import serving
 
     out = []
-    for name, storage, rerank, beam, waves, gather in forms:
+    for name, storage, rerank, beam, waves, gather, screen in forms:
         idx.setPrecision(storage, rerank)
         idx.setBeam(beam)
         idx.setBeamWaves(waves)
         idx.setBeamGather(gather)
+        idx.setScreen(screen)
         idx.setQueryTimeParams({"algoType": "v1merge"})
         srv = serving.Server(idx, lanes=lanes, max_queries=max_queries, max_ef=max_ef,
                              max_k=max(ks), use_graph=True)
@@ -170,6 +171,7 @@ def serve_bench(idx, q, forms, rates, duration_s, lanes=8, max_queries=1024, ef_
     idx.setBeam(0)
     idx.setBeamWaves(0)
     idx.setBeamGather(0)
+    idx.setScreen(0)
     return out
 
 
@@ -178,11 +180,12 @@ def single_query_latency(idx, q, forms, ef, n=32):
     the floor of any served request's latency at that operating point."""
     out = {}
     idx.setQueryTimeParams({"efSearch": ef, "algoType": "v1merge"})
-    for name, storage, rerank, beam, waves, gather in forms:
+    for name, storage, rerank, beam, waves, gather, screen in forms:
         idx.setPrecision(storage, rerank)
         idx.setBeam(beam)
         idx.setBeamWaves(waves)
         idx.setBeamGather(gather)
+        idx.setScreen(screen)
         idx.knnQueryBatchArrays(q[:1], 10)
         ms = []
         for i in range(n):
@@ -196,6 +199,7 @@ def single_query_latency(idx, q, forms, ef, n=32):
     idx.setBeam(0)
     idx.setBeamWaves(0)
     idx.setBeamGather(0)
+    idx.setScreen(0)
     return out
 
 
@@ -218,17 +222,27 @@ FORMS = {
     "bf16-coop32-rerank": ("bf16", 2, 32, 8),
     "bf16-coop48-rerank": ("bf16", 2, 48, 8),
     "bf16-coop64-rerank": ("bf16", 2, 64, 8),
+    # FP32 with the BF16 screen (results identical to the form without it)
+    "fp32-coop48-screen": ("fp32", 0, 48, 8, 1),
+    "fp32-coop64-screen": ("fp32", 0, 64, 8, 1),
+    "fp32-coop64-screen-verify": ("fp32", 0, 64, 8, 2),
 }
+
+
+def form_spec(form):
+    """(storage, rerank multiple, beam width, waves per query, screen)."""
+    return (tuple(FORMS[form]) + (0,))[:5]
 
 
 def measure(idx, q, gids, gd, form, args, dev, log_prefix):
     """Ladder to recall@10 >= 0.95, then a counters run and the timed steps."""
     import torch
 
-    storage, rerank, beam, waves = FORMS[form]
+    storage, rerank, beam, waves, screen = form_spec(form)
     idx.setPrecision(storage, rerank)
     idx.setBeam(beam)
     idx.setBeamWaves(waves)
+    idx.setScreen(screen)
     nq, k = len(q), 10
     dq = torch.from_numpy(q).to(dev)
     dids = torch.empty((nq, k), dtype=torch.int32, device=dev)
@@ -301,11 +315,14 @@ def measure(idx, q, gids, gd, form, args, dev, log_prefix):
     host_qps = nq / (time.time() - t1)
 
     # algorithmic bytes of one launch: adjacency ints read, every distance
-    # evaluation's vector, the rerank's FP32 vectors, queries in, results out
+    # evaluation's vector, the rerank's FP32 vectors, queries in, results out;
+    # under the BF16 screen a screened candidate reads its BF16 vector, and
+    # its FP32 one only if it passed
     dim = q.shape[1]
     vec_bytes = {"fp32": 4, "bf16": 2, "fp8": 1}[storage] * dim + (4 if storage == "fp8" else 0)
     rerank_bytes = nq * min(ef, rerank * k) * dim * 4 if rerank else 0
-    alg_bytes = (st["list_ints_read"] * 4 + (st["dist_level0"] + st["dist_upper"]) * vec_bytes
+    n_vec = st["dist_level0"] + st["dist_upper"] - st["screened"] + st["screen_pass"]
+    alg_bytes = (st["list_ints_read"] * 4 + n_vec * vec_bytes + st["screened"] * 2 * dim
                  + rerank_bytes + nq * dim * 4 + nq * (k * 8 + 4))
     avg_kernel_s = float(np.mean(kernel_ms)) / 1000
     res = {
@@ -318,7 +335,9 @@ def measure(idx, q, gids, gd, form, args, dev, log_prefix):
         "latency_us_p99": round(float(np.percentile(lat, 99)), 1),
         "per_query": {"expansions": round(st["expansions"] / nq, 1),
                       "dist_evals": round((st["dist_level0"] + st["dist_upper"]) / nq, 1),
-                      "tie_path_expansions": round(st["tie_paths"] / nq, 1)},
+                      "tie_path_expansions": round(st["tie_paths"] / nq, 1),
+                      "bf16_screened": round(st["screened"] / nq, 1),
+                      "screen_passed_to_fp32": round(st["screen_pass"] / nq, 1)},
         "alg_bytes_per_launch": int(alg_bytes),
         "achieved_gbs": round(alg_bytes / avg_kernel_s / 1e9, 1),
         "launch": launch,
@@ -328,6 +347,7 @@ def measure(idx, q, gids, gd, form, args, dev, log_prefix):
         tot = sum(phases.values()) or 1.0
         res["step_phase_share"] = {p: round(v / tot, 3) for p, v in phases.items()}
     log(f"{log_prefix} {form}: {res}")
+    idx.setScreen(0)
     return res, ids_np, dists_np
 
 
@@ -430,7 +450,7 @@ def main():
                     "reaching recall@10 >= 0.95)")
     ap.add_argument("--algo", default="v1merge")
     ap.add_argument("--forms", default="fp32-exact,fp32-beam16,bf16-beam16-rerank,fp32-coop64,"
-                                       "bf16-coop64-rerank",
+                                       "fp32-coop64-screen,bf16-coop64-rerank",
                     help=f"comma list of {sorted(FORMS)}; the headline is the fastest")
     ap.add_argument("--nq", type=int, default=10000)
     ap.add_argument("--no-sweep", dest="sweep", action="store_false",
@@ -510,10 +530,11 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         form = args.forms.split(",")[-1]
-        storage, rerank, beam, waves = FORMS[form]
+        storage, rerank, beam, waves, screen = form_spec(form)
         idx.setPrecision(storage, rerank)
         idx.setBeam(beam)
         idx.setBeamWaves(waves)
+        idx.setScreen(screen)
         idx.setQueryTimeParams({"efSearch": args.ef or 4736, "algoType": args.algo})
         ids, dists, counts = idx.knnQueryBatchArrays(q, 10)
         log(f"[bench] pmc pass: copy {nbytes} B, one {form} launch, recall "
@@ -555,13 +576,14 @@ def main():
     if args.serve or args.serve_hi:
         rates = [float(r) for r in args.serve.split(",") if r]
         # (name, storage, rerank multiple, beam width, waves per query, gather:
-        # 2 keeps the cooperative kernel at one block per CU)
-        serve_forms = [("fp32 exact V1Merge", "fp32", 0, 0, 0, 0),
-                       ("fp32 beam16", "fp32", 0, 16, 0, 0),
-                       ("bf16 beam16 + fp32 rerank x2", "bf16", 2, 16, 0, 0),
-                       ("fp32 coop64", "fp32", 0, 64, 8, 0),
-                       ("bf16 coop64 + fp32 rerank x2", "bf16", 2, 64, 8, 0),
-                       ("bf16 coop64 one block per CU + fp32 rerank x2", "bf16", 2, 64, 8, 2)]
+        # 2 keeps the cooperative kernel at one block per CU, BF16 screen)
+        serve_forms = [("fp32 exact V1Merge", "fp32", 0, 0, 0, 0, 0),
+                       ("fp32 beam16", "fp32", 0, 16, 0, 0, 0),
+                       ("bf16 beam16 + fp32 rerank x2", "bf16", 2, 16, 0, 0, 0),
+                       ("fp32 coop64", "fp32", 0, 64, 8, 0, 0),
+                       ("fp32 coop64 + bf16 screen", "fp32", 0, 64, 8, 0, 1),
+                       ("bf16 coop64 + fp32 rerank x2", "bf16", 2, 64, 8, 0, 0),
+                       ("bf16 coop64 one block per CU + fp32 rerank x2", "bf16", 2, 64, 8, 2, 0)]
         if args.serve_forms:
             serve_forms = [f for f in serve_forms if f[0] in args.serve_forms.split(",")]
         mix = serve_bench(idx, q, serve_forms, rates, args.serve_seconds, args.serve_lanes,

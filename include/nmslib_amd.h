@@ -192,6 +192,23 @@ int nmsamd_set_beam_waves(nmsamd_index* h, int waves);
  * (an autotuner knob). */
 int nmsamd_set_beam_gather(nmsamd_index* h, int gather);
 
+/* BF16 screen of the cooperative FP32 beam search on squared l2 (an optimized
+ * index, FP32 storage, setBeamWaves(8) or beam above 16): 1 = with the array
+ * full, each candidate's distance is first bounded from below using the BF16
+ * copy of its vector (a bound proven to hold for the FP32 distance in the
+ * reference's summation order), and only candidates whose bound is below the
+ * admission key load their FP32 vector; the admitted items, their FP32
+ * distances and every result are those of the search without the screen.
+ * 2 = the same, and every candidate's FP32 distance is also computed to check
+ * the bound (a failure is reported as an error).  0 (default) = off.  The first
+ * such search makes the BF16 copy (2 bytes per coordinate).  Ignored by the
+ * other kernels.  No reference counterpart. */
+int nmsamd_set_screen(nmsamd_index* h, int mode);
+
+/* The last search's screen counters (with statistics collection on): out2[0] =
+ * candidates screened, out2[1] = the ones that passed to their FP32 distance. */
+int nmsamd_last_screen_stats(const nmsamd_index* h, uint64_t* out2);
+
 /* Visited marks of the beam-mode searches (the reference keeps a byte per
  * node with a version counter, hnsw.h:568-591): 1 = epoch bytes (n bytes per
  * resident query, cleared when the epoch wraps), 2 = bits (n / 8 bytes,

@@ -76,6 +76,8 @@ SIGNATURES = {
     "nmsamd_set_beam": (C.c_int, [_P, C.c_int]),
     "nmsamd_set_beam_gather": (C.c_int, [_P, C.c_int]),
     "nmsamd_set_visited_marks": (C.c_int, [_P, C.c_int]),
+    "nmsamd_set_screen": (C.c_int, [_P, C.c_int]),
+    "nmsamd_last_screen_stats": (C.c_int, [_P, _P]),
     "nmsamd_set_beam_waves": (C.c_int, [_P, C.c_int]),
     "nmsamd_brute_force": (C.c_int, [_P, _P, _I64, _I64, _I32, _P, _P, _P]),
     "nmsamd_last_phase_us": (C.c_int, [_P, _P]),
@@ -263,6 +265,14 @@ class FloatIndex:
         CU where LDS holds two queries.  Results do not depend on it."""
         _check(load_library().nmsamd_set_beam_gather(self._h, int(gather)))
 
+    def setScreen(self, mode: int = 0):
+        """BF16 screen of the cooperative FP32 l2 beam search: 1 = candidates
+        whose proven lower bound (from their BF16 copy) is at or above the
+        admission key skip their FP32 vector, 2 = the same and every bound is
+        checked against the FP32 distance, 0 = off.  Results do not depend on
+        it."""
+        _check(load_library().nmsamd_set_screen(self._h, int(mode)))
+
     def setVisitedMarks(self, marks: int = 0):
         """Visited marks of beam mode: 1 = epoch bytes, 2 = bits cleared per
         query, 0 = bits where clearing n / 8 bytes is small next to efSearch *
@@ -280,6 +290,9 @@ class FloatIndex:
         keys = ("expansions", "dist_level0", "dist_upper", "list_ints_read", "tie_paths")
         d = {k: int(v) for k, v in zip(keys, out)}
         d["kernel_ms"] = ms.value
+        scr = np.zeros(2, np.uint64)
+        _check(load_library().nmsamd_last_screen_stats(self._h, _ptr(scr)))
+        d["screened"], d["screen_pass"] = int(scr[0]), int(scr[1])
         return d
 
     def lastPhaseUs(self):

@@ -15,7 +15,8 @@ except storage and rerank, whose recall is measured):
   SIMD, two / five / eight);
 * vector storage while traversing: FP32, BF16 or FP8-e4m3 with a per-vector
   scale, the reduced ones followed by an exact FP32 rerank of the best
-  rerank * k candidates (the rerank width).
+  rerank * k candidates (the rerank width);
+* the BF16 screen of the cooperative FP32 l2 search (results unchanged).
 
 ``autotune`` runs a calibration batch -- with its own per-query k and ef --
 in a staged search: every storage x beam width with the kernel chosen from
@@ -31,7 +32,8 @@ import time
 
 import numpy as np
 
-# (name, storage, rerank multiple of k, beam width, waves per query, gather)
+# (name, storage, rerank multiple of k, beam width, waves per query, gather,
+# BF16 screen)
 Form = tuple
 
 STAGE1 = [
@@ -46,6 +48,7 @@ STAGE1 = [
     ("fp8-beam8-rerank2", "fp8", 2, 8, 0, 0),
     ("fp32-coop32", "fp32", 0, 32, 8, 0),
     ("fp32-coop48", "fp32", 0, 48, 8, 0),
+    ("fp32-coop64-screen", "fp32", 0, 64, 8, 0, 1),
     ("bf16-coop32-rerank2", "bf16", 2, 32, 8, 0),
     ("bf16-coop48-rerank2", "bf16", 2, 48, 8, 0),
     ("bf16-coop64-rerank2", "bf16", 2, 64, 8, 0),
@@ -70,16 +73,17 @@ def _ef_bucket(efs):
 
 
 def _apply(index, form):
-    _, storage, rerank, beam, waves, gather = (tuple(form) + (0, 0))[:6]
+    _, storage, rerank, beam, waves, gather, screen = (tuple(form) + (0, 0, 0))[:7]
     index.setPrecision(storage, rerank)
     index.setBeam(beam)
     index.setBeamWaves(waves)
     index.setBeamGather(gather)
+    index.setScreen(screen)
 
 
 def _variants(form):
     """Stage 2: the same form with explicit waves / gather / rerank width."""
-    name, storage, rerank, beam, waves, _ = form
+    name, storage, rerank, beam, waves = form[:5]
     out = []
     if 0 < beam <= 16 and waves != 8:
         out.append((f"{name}-1wave-plain", storage, rerank, beam, 1, 1))
@@ -87,7 +91,8 @@ def _variants(form):
         out.append((f"{name}-2waves", storage, rerank, beam, 2, 0))
         out.append((f"{name}-coop", storage, rerank, beam, 8, 0))
     if waves == 8 or beam > 16:  # one cooperative block per CU, wide distance path
-        out.append((f"{name}-coop-wide", storage, rerank, beam, 8, 2))
+        out.append((f"{name}-coop-wide", storage, rerank, beam, 8, 2,
+                    form[6] if len(form) > 6 else 0))
     if storage != "fp32":
         for r in (1, 4):
             out.append((f"{name.rsplit('-rerank', 1)[0]}-rerank{r}", storage, r, beam, waves, 0))
@@ -107,7 +112,7 @@ def autotune(index, queries, ks, efs, forms=None, recall_slack: float = 0.003, r
         best = _cache[key]
         _apply(index, best["spec"])
         return best
-    stage1 = [tuple(f) + (0, 0)[:6 - len(f)] for f in (forms or STAGE1)]
+    stage1 = [(tuple(f) + (0, 0, 0))[:7] for f in (forms or STAGE1)]
     kmax = int(ks.max())
     gt_ids, gt_d, _ = index.bruteForce(queries, min(128, max(kmax, 10)))
     index.setQueryTimeParams({"algoType": "v1merge"})
@@ -122,7 +127,8 @@ def autotune(index, queries, ks, efs, forms=None, recall_slack: float = 0.003, r
             el = min(el, time.perf_counter() - t)
         launch = index.lastLaunch()
         return {"form": form[0], "spec": form, "storage": form[1], "rerank": form[2],
-                "beam": form[3], "waves": form[4], "gather": form[5], "seconds": el,
+                "beam": form[3], "waves": form[4], "gather": form[5],
+                "screen": form[6] if len(form) > 6 else 0, "seconds": el,
                 "qps": len(queries) / el, "recall": _recall(ids, gt_ids, gt_d, ks),
                 "kernel": launch.get("kernel"), "mode": launch.get("mode")}
 
@@ -144,5 +150,5 @@ def autotune(index, queries, ks, efs, forms=None, recall_slack: float = 0.003, r
         _apply(index, best["spec"])
         _cache[key] = best
     else:
-        _apply(index, ("fp32-exact", "fp32", 0, 0, 0, 0))
+        _apply(index, ("fp32-exact", "fp32", 0, 0, 0, 0, 0))
     return best

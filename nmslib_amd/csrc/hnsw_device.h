@@ -72,6 +72,10 @@ struct SearchStats {
   // (pick, adjacency+visited, distances, filter, sort+prediction, rank search,
   // rebuild, advance)
   unsigned long long phase_ticks[8];
+  // the BF16 screen (cooperative FP32 l2 search): candidates screened, and
+  // the ones that passed to their FP32 distance
+  unsigned long long screened;
+  unsigned long long screen_pass;
 };
 
 struct SearchArgs {
@@ -104,11 +108,15 @@ struct SearchArgs {
   int32_t* old_heap_id;
   int64_t old_heap_cap;
   int* err;                  // bit 1: a SearchOld candidate heap overflowed; bit 2: the
-                             // search state held a node id outside the index or expanded an item twice
+                             // search state held a node id outside the index or expanded an item twice;
+                             // bit 4: the screen's bound failed (verify mode)
   int defer;                 // two-wave beam queries: the first wave rebuilds the sorted
                              // array while the second computes the next step's distances
                              // (needs the second items buffer after the command words)
   int vbits;                 // visited marks as bits (cleared per query), else epoch bytes
+  int screen;                // cooperative FP32 l2 beam search: 1 = screen candidates by a
+                             // lower bound from their BF16 copy, 2 = and verify the bound
+                             // against every FP32 distance (err bit 4 if it failed)
 };
 
 // algoType (hnsw.cc:491-501); hybrid picks SearchOld for ef >= 1000 per query

@@ -2278,6 +2278,103 @@ __device__ __forceinline__ void coop_sort_new(uint2* items, uint2* X, int m, int
   __syncthreads();
 }
 
+// The BF16 screen of the FP32 l2 search (an extension with no reference
+// counterpart; cooperative kernel): a lower bound on the FP32 distance that
+// vec_dist returns, from the BF16 copies q~, x~ of the query and the vector
+// (both rounded to nearest even: |v - v~| <= 2^-8 |v| per coordinate), so a
+// candidate the bound puts at or above the admission key is left out without
+// its FP32 vector: the step admits exactly the items it admits without the
+// screen.  With the FP32 sums sxq ~ q~.x~, sxx ~ |x~|^2 (D/4 dot2 steps per
+// lane, then the pair's halves) and qq ~ |q~|^2, and u = 2^-24:
+//   |q~ - x~|^2 >= (sxx - 2 sxq) + qq - E,  E = (D + 8) 2^-21 (|q~| + |x~|)^2
+//     (the sums err by at most (D/2 + 8) u (|q~| + |x~|)^2 together);
+//   |q - x| >= |q~ - x~| - |q - q~| - |x - x~| >= |q~ - x~| - 2^-8 / (1 - 2^-8)
+//     (|q~| + |x~|), norms from sxx and qq enlarged by 2^-10 for their error;
+//   vec_dist's FP32 sum of D non-negative rounded squares >= |q - x|^2
+//     (1 - (D + 2) u).
+// Each step is taken with slack for the FP32 rounding of the bound itself
+// (the square root shrunk by 1e-5, the result by (D + 8) u + 2^-20).  A bound
+// that is not a number never rejects.
+template <int D>
+__device__ __forceinline__ float screen_bound(const VecRegs<D, kBF16>& r, const float* qs, int h,
+                                              int dstride) {
+  constexpr int Q16 = D / 16;
+  const uint32_t* qb = query_bf16(qs, dstride);
+  float sxq = 0.f, sxx = 0.f;
+#pragma unroll
+  for (int c = 0; c < Q16; ++c) {
+    const uint4 xr = r.c[c];
+    const uint4 qr = *reinterpret_cast<const uint4*>(qb + 8 * c + 4 * h);
+    sxq = dot2_bf16(xr.x, qr.x, sxq);
+    sxq = dot2_bf16(xr.y, qr.y, sxq);
+    sxq = dot2_bf16(xr.z, qr.z, sxq);
+    sxq = dot2_bf16(xr.w, qr.w, sxq);
+    sxx = dot2_bf16(xr.x, xr.x, sxx);
+    sxx = dot2_bf16(xr.y, xr.y, sxx);
+    sxx = dot2_bf16(xr.z, xr.z, sxx);
+    sxx = dot2_bf16(xr.w, xr.w, sxx);
+  }
+#pragma unroll
+  for (int t = 0; 2 * t + h < D % 16; ++t) {
+    const int c = 16 * Q16 + 2 * t + h;
+    const uint32_t xw = r.t[t];
+    const uint32_t qw = (qb[c >> 1] >> (16 * (c & 1))) & 0xffffu;
+    sxq = dot2_bf16(xw, qw, sxq);
+    sxx = dot2_bf16(xw, xw, sxx);
+  }
+  sxq = sxq + __shfl_xor(sxq, 1);
+  sxx = sxx + __shfl_xor(sxx, 1);
+  const float qq = reinterpret_cast<const float*>(qb)[dstride / 2];
+  const float nsum = (sqrtf(fmaxf(sxx, 0.f)) + sqrtf(fmaxf(qq, 0.f))) * (1.0f + 0x1p-10f);
+  const float e = (float)(D + 8) * 0x1p-21f * nsum * nsum;
+  const float dlo = ((sxx - 2.0f * sxq) + qq) - e;
+  const float rr = dlo > 0.f ? sqrtf(dlo) * 0.99999f : 0.f;
+  const float rp = rr - (nsum * (0x1p-8f / (1.0f - 0x1p-8f)) * 1.00001f + 1e-30f);
+  const float c3 = (float)(D + 8) * 0x1p-24f + 0x1p-20f;
+  return rp > 0.f ? (rp * rp) * (1.0f - c3) : (rp == rp ? 0.f : rp);
+}
+
+// Screens the wave's candidates cand[s0..s1) against the admission key topk
+// (the array full): the ones whose bound stays below topk are compacted, in
+// order, to cand[s0..s0 + returned).  Two BF16 vectors per lane pair in
+// flight.  verify: also computes each candidate's FP32 distance and sets bit
+// 4 of the error word if a left-out one would have been admitted.
+template <int D, int FUNC>
+__device__ __forceinline__ int screen_slice(const SearchArgs& a, const WaveLds& w, int s0, int s1,
+                                            float topk, int lane, int h, int fallback, bool verify) {
+  const DevIndex& ix = a.ix;
+  uint64_t ltm = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  int kept = 0;
+  for (int b = s0; b < s1; b += 64) {
+    VecRegs<D, kBF16> r[2];
+    int id[2];
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const int s = b + 32 * v + (lane >> 1);
+      id[v] = checked_node(s < s1 ? w.cand[s] : fallback, (int)ix.n, a.err);
+      vec_load<D, kBF16>(r[v], ix, id[v], h);
+    }
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const int s = b + 32 * v + (lane >> 1);
+      const float lb = screen_bound<D>(r[v], w.qs, h, ix.dstride);
+      const bool keep = !(lb >= topk);
+      if (verify) {
+        const float d = dist_node<D, FUNC, kF32>(w.qs, ix, id[v], h);
+        if (s < s1 && !keep && d < topk && a.err) atomicOr(a.err, 4);
+      }
+      // lane 2j (h == 0) speaks for candidate s; the compaction keeps order
+      const bool k0 = s < s1 && keep && h == 0;
+      const uint64_t km = __ballot(k0);
+      WSYNC();  // every read of this round's ids is done before the writes
+      if (k0) w.cand[s0 + kept + __popcll(km & ltm)] = id[v];
+      kept += __popcll(km);
+    }
+  }
+  WSYNC();
+  return kept;
+}
+
 template <int D, int FUNC, int ST, int DPATH>
 __device__ __forceinline__ void coop_query(const SearchArgs& a, const WaveLds& w, const Marks& vis,
                                            int qi, int k, int ef, int wave, int lane, int h,
@@ -2299,6 +2396,9 @@ __device__ __forceinline__ void coop_query(const SearchArgs& a, const WaveLds& w
   int* lrank = reinterpret_cast<int*>(w.items);
   uint32_t* vw = reinterpret_cast<uint32_t*>(vis.p);
   unsigned long long nexp = 0, nd0 = 0, ndu = 1, nl = 0;
+  // the BF16 screen: FP32 l2 on a specialised dimension
+  constexpr bool kScreen = ST == kF32 && (FUNC == kL2Sqr16 || FUNC == kL2Sqr) && D > 0 && D % 4 == 0;
+  unsigned long long nscr = 0, npass = 0;  // this wave's screened candidates, and passed
 
   if (wave == 0) {
     int cur = ix.enter;
@@ -2425,15 +2525,25 @@ __device__ __forceinline__ void coop_query(const SearchArgs& a, const WaveLds& w
     nd0 += ncand;
 
     // 3. distances of a slice of the candidates per wave, and the admission
+    // (with the array full, the BF16 screen first leaves out the candidates
+    // its bound proves above topk)
     const int per = (ncand + kCoopWaves - 1) / kCoopWaves;
     const int s0 = min(ncand, wave * per), s1 = min(ncand, s0 + per);
-    dist_cands<D, FUNC, ST, DPATH>(w.qs, ix, w.cand + s0, w.cdist + s0, s1 - s0, lane, h, fallback,
+    int se = s1;
+    if constexpr (kScreen) {
+      if (a.screen && nsnap >= ef) {
+        se = s0 + screen_slice<D, FUNC>(a, w, s0, s1, topk, lane, h, fallback, a.screen == 2);
+        nscr += s1 - s0;
+        npass += se - s0;
+      }
+    }
+    dist_cands<D, FUNC, ST, DPATH>(w.qs, ix, w.cand + s0, w.cdist + s0, se - s0, lane, h, fallback,
                                    a.err);
     WSYNC();
-    for (int b = s0; b < s1; b += 64) {
+    for (int b = s0; b < se; b += 64) {
       const int s = b + lane;
-      const float dj = s < s1 ? w.cdist[s] : 0.f;
-      const bool keep = s < s1 && (dj < topk || nsnap < ef);
+      const float dj = s < se ? w.cdist[s] : 0.f;
+      const bool keep = s < se && (dj < topk || nsnap < ef);
       const uint64_t km = __ballot(keep);
       int base = 0;
       if (lane == 0 && km) base = atomicAdd(&ctl[1], __popcll(km));
@@ -2542,6 +2652,10 @@ __device__ __forceinline__ void coop_query(const SearchArgs& a, const WaveLds& w
   }
   if (timed)
     for (int p = 0; p < 8; ++p) atomicAdd(&a.stats->phase_ticks[p], (unsigned long long)ctl[10 + p]);
+  if (kScreen && a.stats && lane == 0 && nscr) {
+    atomicAdd(&a.stats->screened, nscr);
+    atomicAdd(&a.stats->screen_pass, npass);
+  }
   if (wave == 0) {
     const QueryCounters c{nexp, nd0, ndu, nl + (unsigned long long)ctl[6], 0};
     finish_query<D, FUNC, ST>(a, w, n, qi, k, lane, h, c);
@@ -2593,7 +2707,7 @@ __device__ __forceinline__ void coop_body(const SearchArgs& a) {
           for (int i = lane; i < dim; i += 64) w.qs[i] = w.qs[i] * inv;
         WSYNC();
       }
-      if constexpr (ST == kBF16) make_query_bf16(w.qs, a.ix.dstride, lane);
+      if (ST == kBF16 || a.screen) make_query_bf16(w.qs, a.ix.dstride, lane);
     }
     __syncthreads();
     if (k <= 0 || a.ix.n == 0) {

@@ -109,6 +109,7 @@ struct nmsamd_index {
   int beam_waves = 0;  // waves per beam query: 0 = by LDS footprint, 1 or 2
   int beam_gather = 0;  // one-wave beam kernel: 0 = by LDS footprint, 1 = plain, 2 = wide
   int visited_marks = 0;  // 0 = by size (use_bit_marks), 1 = epoch bytes, 2 = bits
+  int screen = 0;  // cooperative FP32 l2 beam search: 1 = BF16 screen, 2 = and verify
   bool slots_bits = false;  // the last launch over d_visited left bit marks in it
   bool collect = false;
   // file header (hnsw.cc:779-788)
@@ -528,9 +529,9 @@ bool use_bit_marks(const nmsamd_index* h, int max_ef, bool beam) {
 // The reduced-precision copy the current storage traverses, converted on the
 // first use.  The conversion finishes before this returns: searches and
 // serving lanes on other streams read the copy right after.
-void ensure_lowp(nmsamd_index* h) {
+void ensure_lowp(nmsamd_index* h, bool need16 = false) {
   bool converted = false;
-  if (h->storage == kBF16 && !h->d_vec16.p) {
+  if ((h->storage == kBF16 || need16) && !h->d_vec16.p) {
     h->d_vec16.ensure((size_t)std::max<int64_t>(h->n, 1) * h->dstride);
     hip_check(launch_convert_lowp(h->d_vec.p, h->d_vec16.p, nullptr, nullptr, h->n, h->dim,
                                   h->dstride, kBF16, h->stream), "convert bf16");
@@ -579,6 +580,11 @@ bool use_coop(const nmsamd_index* h) { return h->beam > 0 && (h->beam_waves == 8
 // Two cooperative blocks per CU (kModeBeamCoop2) where LDS holds two queries,
 // unless setBeamGather(2) asks for the wide distance path of one per CU.
 bool coop_two_per_cu(const nmsamd_index* h, size_t fit) { return fit >= 2 && h->beam_gather != 2; }
+// The BF16 screen where it applies: the cooperative kernel on FP32 vectors,
+// squared l2 of an optimized index (SearchArgs::screen).
+int screen_mode(const nmsamd_index* h) {
+  return (h->storage == kF32 && (h->func == kL2Sqr16 || h->func == kL2Sqr)) ? h->screen : 0;
+}
 
 // Neighbour entries one search step holds in LDS: beam mode takes up to
 // kMaxStepEntries / maxM0 expansions per step (kMaxCoopEntries on the
@@ -693,12 +699,13 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   if ((size_t)lds > kLdsPerCu)
     fail(NMSAMD_EINVAL, "efSearch / k too large: the search array of " + std::to_string(cap) +
                             " items exceeds the LDS of one wave");
+  const int screen = coop ? screen_mode(h) : 0;
   if (ext) {
-    if ((h->storage == kBF16 && !h->d_vec16.p) || (h->storage == kFP8 && !h->d_vec8.p))
+    if (((h->storage == kBF16 || screen) && !h->d_vec16.p) || (h->storage == kFP8 && !h->d_vec8.p))
       fail(NMSAMD_EINVAL, "the reduced-precision copy of the vectors is made by the first "
                           "search in that precision: run one through another entry point first");
   } else {
-    ensure_lowp(h);
+    ensure_lowp(h, screen != 0);
   }
   // queries per block and resident blocks per CU from the LDS footprint; the
   // grid is persistent: as many resident queries as the device holds, at most
@@ -833,6 +840,7 @@ void run_search(nmsamd_index* h, const float* d_q, int64_t nq, int k, int ef, co
   a.old_heap_cap = heap_cap;
   a.defer = defer ? 1 : 0;
   a.vbits = vbits ? 1 : 0;
+  a.screen = screen;
   hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
   hip_check(launch_search(a, blocks, wpb, lds_launch, mode, s), "search kernel launch");
   hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
@@ -853,6 +861,8 @@ void finish_stats(nmsamd_index* h) {
   hip_check(hipMemcpy(&err, h->last_err, sizeof(int), hipMemcpyDeviceToHost), "status");
   if (err & 2)
     fail(NMSAMD_ERUNTIME, "internal error: corrupted search state (a node id outside the index, or an item expanded twice)");
+  if (err & 4)
+    fail(NMSAMD_ERUNTIME, "internal error: the BF16 screen's bound failed against an FP32 distance");
   if (err)
     fail(NMSAMD_ERUNTIME, "SearchOld candidate heap overflow (more than 64*ef admitted items)");
   float ms = 0;
@@ -1375,6 +1385,25 @@ int nmsamd_set_visited_marks(nmsamd_index* h, int marks) {
   });
 }
 
+int nmsamd_set_screen(nmsamd_index* h, int mode) {
+  return guarded([&] {
+    if (!h) fail(NMSAMD_EINVAL, "null handle");
+    if (mode < 0 || mode > 2)
+      fail(NMSAMD_EINVAL, "screen must be 0 (off), 1 (BF16 screen) or 2 (screen and verify)");
+    h->screen = mode;
+  });
+}
+
+int nmsamd_last_screen_stats(const nmsamd_index* hc, uint64_t* out2) {
+  return guarded([&] {
+    if (!hc || !out2) fail(NMSAMD_EINVAL, "null argument");
+    auto* h = const_cast<nmsamd_index*>(hc);
+    if (h->ev1 && hipEventQuery(h->ev1) == hipSuccess) finish_stats(h);
+    out2[0] = h->last_stats.screened;
+    out2[1] = h->last_stats.screen_pass;
+  });
+}
+
 int nmsamd_set_beam_waves(nmsamd_index* h, int waves) {
   return guarded([&] {
     if (!h) fail(NMSAMD_EINVAL, "null handle");
@@ -1485,6 +1514,7 @@ struct nmsamd_server {
   int vbits = 0;  // visited marks as bits
   int coop = 0;   // cooperative kernel (one query per block of kCoopWaves waves)
   int coop2 = 0;  // two cooperative blocks per CU
+  int screen = 0;  // BF16 screen of the cooperative FP32 l2 search (SearchArgs::screen)
   DevIndex ix{};
   int slots_per_lane = 0;
   bool use_graph = false;
@@ -1529,6 +1559,7 @@ SearchArgs server_args(nmsamd_server* s, ServerLane& L) {
   a.slot_base = L.slot_base;
   a.defer = s->defer;
   a.vbits = s->vbits;
+  a.screen = s->screen;
   return a;
 }
 
@@ -1570,9 +1601,10 @@ int nmsamd_server_open(nmsamd_index* h, int lanes, int max_queries, int max_ef, 
                                     std::min(std::max(max_ef, max_k), h->rerank_mult * max_k));
     s->ent = step_entries(h);
     s->coop = use_coop(h) ? 1 : 0;
+    s->screen = s->coop ? screen_mode(h) : 0;
     s->lds = lds_wave_bytes(s->cap, h->dstride, s->ent, h->beam > 0);
     if ((size_t)s->lds > kLdsPerCu) fail(NMSAMD_EINVAL, "max_ef / max_k exceed the LDS of one wave");
-    ensure_lowp(h);  // finished on return: the lanes' streams read the copy at once
+    ensure_lowp(h, s->screen != 0);  // finished on return: the lanes' streams read the copy at once
     s->storage = h->storage;
     s->rerank = (h->storage != kF32 && h->rerank_mult > 0) ? h->rerank_mult : 0;
     s->beam = h->beam;
@@ -1708,6 +1740,8 @@ int nmsamd_server_fetch(nmsamd_server* s, int lane, int32_t* ids, float* dists, 
     L.busy = false;
     if (*L.herr & 2)
       fail(NMSAMD_ERUNTIME, "internal error: corrupted search state (a node id outside the index, or an item expanded twice)");
+    if (*L.herr & 4)
+      fail(NMSAMD_ERUNTIME, "internal error: the BF16 screen's bound failed against an FP32 distance");
     const size_t out = (size_t)L.nq * s->max_k;
     std::memcpy(ids, L.hids, sizeof(int32_t) * out);
     std::memcpy(dists, L.hdists, sizeof(float) * out);

@@ -96,6 +96,36 @@ def test_fullsize_beam_search_equals_model(big, beam, waves):
     np.testing.assert_array_equal(counts, oc)
 
 
+@pytest.mark.parametrize("beam,screen", [(64, 2), (48, 1)])
+def test_fullsize_screened_beam_search_equals_model(big, beam, screen):
+    # the BF16 screen of the cooperative FP32 search leaves out candidates
+    # whose proven lower bound reaches the admission key: the rows equal the
+    # FP32 beam model's; screen 2 also checks every bound against the FP32
+    # distance on the device (a failure raises)
+    path, idx, q, _, _ = big
+    idx.setBeam(beam)
+    idx.setBeamWaves(8)
+    idx.setScreen(screen)
+    idx.setPrecision("fp32", 0)
+    idx.setQueryTimeParams({"efSearch": 4736, "algoType": "v1merge"})
+    idx.collectStats(True)
+    try:
+        ids, dists, counts = idx.knnQueryBatchArrays(q[:48], 10)
+        st = idx.lastStats()
+    finally:
+        idx.collectStats(False)
+        idx.setScreen(0)
+        idx.setBeam(0)
+        idx.setBeamWaves(0)
+    print(f"beam {beam} screen {screen}: screened {st['screened']} passed {st['screen_pass']} "
+          f"of {st['dist_level0']}", flush=True)
+    assert st["screened"] > st["dist_level0"] // 2 and st["screen_pass"] < st["screened"]
+    oi, od, oc = O.OracleIndex(path).search_beam(q[:48], 10, 4736, beam, num_threads=8)
+    np.testing.assert_array_equal(ids, oi)
+    np.testing.assert_array_equal(dists.view(np.uint32), od.view(np.uint32))
+    np.testing.assert_array_equal(counts, oc)
+
+
 def test_fullsize_bruteforce_equals_ground_truth(big):
     _, idx, q, gids, gd = big
     ids, dists, counts = idx.bruteForce(q[:64], 100)

@@ -274,6 +274,45 @@ def test_gpu_coop_beam_equals_model(name, beam, gather):
     idx.setBeam(0)
 
 
+@pytest.mark.parametrize("name", ["u128_l2", "hn128_l2", "u10k_l2"])
+@pytest.mark.parametrize("beam", [4, 16, 64])
+@pytest.mark.parametrize("screen,gather", [(1, 0), (2, 0), (2, 2)])
+def test_gpu_coop_screen_equals_model(name, beam, screen, gather):
+    # the BF16 screen of the cooperative FP32 l2 search: rows, distances and
+    # counters equal the FP32 beam model's; screen 2 checks every bound
+    # against the FP32 distance on the device (a failure raises)
+    m = F.META[name]
+    idx = load(name, F.index_path(name), m["space"])
+    oracle = O.OracleIndex(F.index_path(name))
+    q = F.results(name)["queries"]
+    idx.setBeam(beam)
+    idx.setBeamWaves(8)
+    idx.setBeamGather(gather)
+    idx.setScreen(screen)
+    idx.collectStats(True)
+    screened = 0
+    for k, ef in [(10, 16), (10, 64), (1, 300)]:
+        idx.setQueryTimeParams({"efSearch": ef, "algoType": "v1merge"})
+        ids, dists, counts = idx.knnQueryBatchArrays(q, k)
+        st = idx.lastStats()
+        screened += st["screened"]
+        assert st["screen_pass"] <= st["screened"] <= st["dist_level0"]
+        rids, rdists, rcounts, rst = oracle.search_beam(q, k, ef, beam, num_threads=8,
+                                                        with_stats=True)
+        msg = f"{name} beam={beam} screen={screen} gather={gather} k={k} ef={ef}"
+        print(msg, st["screened"], st["screen_pass"], flush=True)
+        np.testing.assert_array_equal(counts, rcounts, err_msg=msg)
+        np.testing.assert_array_equal(ids, rids, err_msg=msg)
+        np.testing.assert_array_equal(dists.view(np.uint32), rdists.view(np.uint32), err_msg=msg)
+        for key in ("expansions", "dist_level0", "dist_upper", "list_ints_read"):
+            assert st[key] == rst[key], (msg, key, st[key], rst[key])
+    assert screened > 0
+    idx.setScreen(0)
+    idx.setBeamGather(0)
+    idx.setBeamWaves(0)
+    idx.setBeam(0)
+
+
 def test_gpu_bf16_on_integer_data_is_exact():
     # coordinates 0..255 are exact in BF16 and every partial sum is an exact
     # integer below 2^24, so BF16 traversal reproduces the FP32 results

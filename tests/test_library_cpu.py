@@ -93,6 +93,11 @@ def test_extension_setters_validate_their_arguments():
         with pytest.raises(ValueError, match="beam waves must be"):
             idx.setBeamWaves(bad)
     for v in (0, 1, 2):
+        idx.setScreen(v)
+    for bad in (-1, 3):
+        with pytest.raises(ValueError, match="screen must be"):
+            idx.setScreen(bad)
+    for v in (0, 1, 2):
         idx.setVisitedMarks(v)
     for bad in (-1, 3):
         with pytest.raises(ValueError, match="visited marks must be"):
