@@ -82,17 +82,21 @@ def test_fullsize_server_two_wave_lanes_at_the_recall_point(big):
     idx.setBeam(0)
 
 
-@pytest.mark.parametrize("storage,rerank", [("fp32", 0), ("bf16", 2)])
-def test_fullsize_server_coop_lanes_at_the_recall_point(big, storage, rerank):
+@pytest.mark.parametrize("storage,rerank,gather,screen", [("fp32", 0, 0, 0), ("fp32", 0, 0, 1),
+                                                          ("bf16", 2, 0, 0), ("bf16", 2, 2, 0)])
+def test_fullsize_server_coop_lanes_at_the_recall_point(big, storage, rerank, gather, screen):
     # every request at efSearch 4736 on the cooperative kernel (beam 64, a
-    # block of eight waves per query): the served rows equal the batch entry
+    # block of eight waves per query; two blocks per CU, or one with gather 2;
+    # FP32 also with the BF16 screen): the served rows equal the batch entry
     # point's for the same requests
     idx, pool = big
     idx.setPrecision(storage, rerank)
     idx.setBeam(64)
     idx.setBeamWaves(8)
+    idx.setBeamGather(gather)
+    idx.setScreen(screen)
     srv = serving.Server(idx, lanes=4, max_queries=256, max_ef=4736, max_k=10, use_graph=True)
-    assert srv.info()["beam_kernel"] == 3
+    assert srv.info()["beam_kernel"] == (3 if gather == 2 else 4)
     reqs = serving.make_requests(len(pool), rate_qps=20_000, duration_s=0.3, seed=13,
                                  size_range=(1, 64), ks=(10,), ef_range=(4736, 4736))
     st = serving.run_poisson(srv, pool, reqs, keep_results=True)
@@ -107,5 +111,7 @@ def test_fullsize_server_coop_lanes_at_the_recall_point(big, storage, rerank):
         np.testing.assert_array_equal(gi, ids[:, :r.k], err_msg=f"request {r.rid}")
         np.testing.assert_array_equal(gd, dists[:, :r.k], err_msg=f"request {r.rid}")
     idx.setPrecision("fp32", 0)
+    idx.setScreen(0)
+    idx.setBeamGather(0)
     idx.setBeamWaves(0)
     idx.setBeam(0)
