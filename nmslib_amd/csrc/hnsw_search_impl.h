@@ -2336,26 +2336,26 @@ __device__ __forceinline__ float screen_bound(const VecRegs<D, kBF16>& r, const 
 
 // Screens the wave's candidates cand[s0..s1) against the admission key topk
 // (the array full): the ones whose bound stays below topk are compacted, in
-// order, to cand[s0..s0 + returned).  Two BF16 vectors per lane pair in
+// order, to cand[s0..s0 + returned).  NV BF16 vectors per lane pair in
 // flight.  verify: also computes each candidate's FP32 distance and sets bit
 // 4 of the error word if a left-out one would have been admitted.
-template <int D, int FUNC>
+template <int D, int FUNC, int NV>
 __device__ __forceinline__ int screen_slice(const SearchArgs& a, const WaveLds& w, int s0, int s1,
                                             float topk, int lane, int h, int fallback, bool verify) {
   const DevIndex& ix = a.ix;
   uint64_t ltm = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   int kept = 0;
-  for (int b = s0; b < s1; b += 64) {
-    VecRegs<D, kBF16> r[2];
-    int id[2];
+  for (int b = s0; b < s1; b += 32 * NV) {
+    VecRegs<D, kBF16> r[NV];
+    int id[NV];
 #pragma unroll
-    for (int v = 0; v < 2; ++v) {
+    for (int v = 0; v < NV; ++v) {
       const int s = b + 32 * v + (lane >> 1);
       id[v] = checked_node(s < s1 ? w.cand[s] : fallback, (int)ix.n, a.err);
       vec_load<D, kBF16>(r[v], ix, id[v], h);
     }
 #pragma unroll
-    for (int v = 0; v < 2; ++v) {
+    for (int v = 0; v < NV; ++v) {
       const int s = b + 32 * v + (lane >> 1);
       const float lb = screen_bound<D>(r[v], w.qs, h, ix.dstride);
       const bool keep = !(lb >= topk);
@@ -2532,7 +2532,10 @@ __device__ __forceinline__ void coop_query(const SearchArgs& a, const WaveLds& w
     int se = s1;
     if constexpr (kScreen) {
       if (a.screen && nsnap >= ef) {
-        se = s0 + screen_slice<D, FUNC>(a, w, s0, s1, topk, lane, h, fallback, a.screen == 2);
+        // one BF16 vector per lane pair in flight at four waves per SIMD (two
+        // spill), two at two waves per SIMD
+        se = s0 + screen_slice<D, FUNC, DPATH == 1 ? 2 : 1>(a, w, s0, s1, topk, lane, h, fallback,
+                                                             a.screen == 2);
         nscr += s1 - s0;
         npass += se - s0;
       }
