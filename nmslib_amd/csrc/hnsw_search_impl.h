@@ -2211,9 +2211,10 @@ __device__ __forceinline__ void beam_helper(const SearchArgs& a, const WaveLds& 
 // the beam model's:
 //   1. picks and the first unexpanded item left (wave 0);
 //   2. adjacency rows, visited marks and compaction, the entries split across
-//      all threads: a mark is a bit set by an atomic OR that returns the old
-//      word, so the probe, the mark and the dedupe across the picks' lists are
-//      one operation (two copies of an id: one sees the bit clear);
+//      all threads: a mark is a bit, read from L2 first; where it is clear, an
+//      atomic OR that returns the old word sets it, so the mark and the dedupe
+//      across the picks' lists are one operation (two copies of an id: one
+//      sees the bit clear);
 //   3. distances (each wave a slice of the candidates, wide gathers) and the
 //      snapshot admission, items placed through an LDS counter;
 //   4. the rank of every new item among the new items (one item per thread:
@@ -2395,10 +2396,11 @@ __device__ __forceinline__ void coop_query(const SearchArgs& a, const WaveLds& w
   uint2* X = reinterpret_cast<uint2*>(w.cand);
   int* lrank = reinterpret_cast<int*>(w.items);
   uint32_t* vw = reinterpret_cast<uint32_t*>(vis.p);
-  unsigned long long nexp = 0, nd0 = 0, ndu = 1, nl = 0;
+  unsigned long long ndu = 1, nl = 0;  // wave 0: the descent's counters and rows read
+  uint32_t nexp = 0, nd0 = 0;           // expansions and level-0 distances of the query
   // the BF16 screen: FP32 l2 on a specialised dimension
   constexpr bool kScreen = ST == kF32 && (FUNC == kL2Sqr16 || FUNC == kL2Sqr) && D > 0 && D % 4 == 0;
-  unsigned long long nscr = 0, npass = 0;  // this wave's screened candidates, and passed
+  uint32_t nscr = 0, npass = 0;  // this wave's screened candidates, and passed
 
   if (wave == 0) {
     int cur = ix.enter;
@@ -2500,10 +2502,19 @@ __device__ __forceinline__ void coop_query(const SearchArgs& a, const WaveLds& w
       }
     }
     uint32_t old[RMAX];
+    // the mark words read from L2 first: about half the entries are visited
+    // already, and only the others take the atomic OR (whose old word decides,
+    // so two copies of an id in this step still yield one fresh entry)
+    uint32_t pre[RMAX];
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r)
+      pre[r] = t[r] >= 0 ? __hip_atomic_load(vw + (t[r] >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                         : 0xffffffffu;
 #pragma unroll
     for (int r = 0; r < RMAX; ++r) {
       old[r] = 0xffffffffu;
-      if (t[r] >= 0) old[r] = atomicOr(vw + (t[r] >> 5), 1u << (t[r] & 31));
+      if (t[r] >= 0 && !((pre[r] >> (t[r] & 31)) & 1u))
+        old[r] = atomicOr(vw + (t[r] >> 5), 1u << (t[r] & 31));
     }
     int lnl = 0;
 #pragma unroll
@@ -2656,8 +2667,8 @@ __device__ __forceinline__ void coop_query(const SearchArgs& a, const WaveLds& w
   if (timed)
     for (int p = 0; p < 8; ++p) atomicAdd(&a.stats->phase_ticks[p], (unsigned long long)ctl[10 + p]);
   if (kScreen && a.stats && lane == 0 && nscr) {
-    atomicAdd(&a.stats->screened, nscr);
-    atomicAdd(&a.stats->screen_pass, npass);
+    atomicAdd(&a.stats->screened, (unsigned long long)nscr);
+    atomicAdd(&a.stats->screen_pass, (unsigned long long)npass);
   }
   if (wave == 0) {
     const QueryCounters c{nexp, nd0, ndu, nl + (unsigned long long)ctl[6], 0};
