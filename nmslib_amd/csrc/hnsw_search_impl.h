@@ -321,6 +321,12 @@ __device__ __forceinline__ float lp_pair(float acc) {
 // (v_dot2_f32_bf16, two coordinates per instruction, FP32 accumulation).
 // The rounded query sits right after the FP32 one: word i of qb = coordinates
 // 2i and 2i+1, and qb[dstride / 2] = the sum of its squares.
+// s_waitcnt vmcnt(0): every vector memory operation of this wave, stores and
+// atomics without return included, has been performed (a workgroup-scope
+// release fence does not wait for them, and an agent-scope one also writes the
+// L2 back, which marks that only this CU reads do not need)
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 __device__ __forceinline__ const uint32_t* query_bf16(const float* qs, int dstride) {
   return reinterpret_cast<const uint32_t*>(qs + dstride);
 }
@@ -2501,7 +2507,6 @@ __device__ __forceinline__ void coop_query(const SearchArgs& a, const WaveLds& w
         t[r] = ix.l0[(int64_t)checked_node(picks[li], (int)ix.n, a.err) * ix.l0stride + sl];
       }
     }
-    uint32_t old[RMAX];
     // the mark words read from L2 first: about half the entries are visited
     // already, and only the others take the atomic OR (whose old word decides,
     // so two copies of an id in this step still yield one fresh entry)
@@ -2510,6 +2515,7 @@ __device__ __forceinline__ void coop_query(const SearchArgs& a, const WaveLds& w
     for (int r = 0; r < RMAX; ++r)
       pre[r] = t[r] >= 0 ? __hip_atomic_load(vw + (t[r] >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                          : 0xffffffffu;
+    uint32_t old[RMAX];
 #pragma unroll
     for (int r = 0; r < RMAX; ++r) {
       old[r] = 0xffffffffu;
@@ -2707,7 +2713,8 @@ __device__ __forceinline__ void coop_body(const SearchArgs& a) {
     for (int64_t i = threadIdx.x; i < nv4; i += NT) v4[i] = make_uint4(0, 0, 0, 0);
     const float* qg = a.queries + (int64_t)qi * dim;
     for (int i = threadIdx.x; i < a.ix.dstride; i += NT) w.qs[i] = i < dim ? qg[i] : 0.f;
-    __syncthreads();  // with its release fence: the clearing stores are in L2 before any OR
+    wait_vmem();      // the clearing stores are performed in L2 before any wave's OR
+    __syncthreads();
     if (wave == 0) {
       if constexpr (FUNC == kNormCos) {
         float inv = 0.f;
