@@ -1,5 +1,6 @@
-"""The reference-side plugin of INTEGRATION.md (an Index<float> method that
-forwards to the C ABI, similarity_search/include/index.h:50-109) compiles
+"""The reference-side plugin of the repository's integration notes (an
+Index<float> method that forwards to the C ABI,
+similarity_search/include/index.h:50-109) compiles
 against the reference's own headers and include/nmslib_amd.h.  Runs where the
 reference sources are present; the code blocks are taken from the document
 itself, so what it shows is what is compiled."""
