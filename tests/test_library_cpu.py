@@ -97,6 +97,9 @@ def test_extension_setters_validate_their_arguments():
     for bad in (-1, 3):
         with pytest.raises(ValueError, match="screen must be"):
             idx.setScreen(bad)
+    # the screen's counters read through their own entry point, zero before a search
+    st = idx.lastStats()
+    assert st["screened"] == 0 and st["screen_pass"] == 0
     for v in (0, 1, 2):
         idx.setVisitedMarks(v)
     for bad in (-1, 3):
