@@ -343,19 +343,31 @@ __device__ __forceinline__ float dot2_bf16(uint32_t a, uint32_t b, float c) {
                                          __builtin_bit_cast(bf16x2_t, b), c, false);
 }
 
-// Writes the BF16 query (and its squared norm) after qs[0..dstride); one wave.
+// Writes the BF16 query, its squared norm and the norm of its rounding error
+// |q - q~| (the differences exact in FP32, their squares summed in FP64,
+// rounded up: the BF16 screen's bound) after qs[0..dstride); one wave.
 __device__ __forceinline__ void make_query_bf16(float* qs, int dstride, int lane) {
   uint32_t* qb = reinterpret_cast<uint32_t*>(qs + dstride);
   float part = 0.f;
+  double e2 = 0.0;
   for (int i = lane; i < dstride / 2; i += 64) {
     const uint32_t w = bf16_rne(qs[2 * i]) | (bf16_rne(qs[2 * i + 1]) << 16);
     qb[i] = w;
     part = dot2_bf16(w, w, part);
+    const double d0 = (double)qs[2 * i] - (double)__uint_as_float(w << 16);
+    const double d1 = (double)qs[2 * i + 1] - (double)__uint_as_float(w & 0xffff0000u);
+    e2 += d0 * d0 + d1 * d1;
   }
   // fixed butterfly order: every wave of every kernel gets the same sum
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) part = part + __shfl_xor(part, o);
-  if (lane == 0) reinterpret_cast<float*>(qb)[dstride / 2] = part;
+  for (int o = 32; o > 0; o >>= 1) {
+    part = part + __shfl_xor(part, o);
+    e2 += __shfl_xor(e2, o);
+  }
+  if (lane == 0) {
+    reinterpret_cast<float*>(qb)[dstride / 2] = part;
+    reinterpret_cast<float*>(qb)[dstride / 2 + 1] = (float)(sqrt(e2) * (1.0 + 0x1p-20)) + 1e-37f;
+  }
 }
 
 // The BF16 distance from the lane's partial products: x.q over its
@@ -2295,8 +2307,9 @@ __device__ __forceinline__ void coop_sort_new(uint2* items, uint2* X, int m, int
 // lane, then the pair's halves) and qq ~ |q~|^2, and u = 2^-24:
 //   |q~ - x~|^2 >= (sxx - 2 sxq) + qq - E,  E = (D + 8) 2^-21 (|q~| + |x~|)^2
 //     (the sums err by at most (D/2 + 8) u (|q~| + |x~|)^2 together);
-//   |q - x| >= |q~ - x~| - |q - q~| - |x - x~| >= |q~ - x~| - 2^-8 / (1 - 2^-8)
-//     (|q~| + |x~|), norms from sxx and qq enlarged by 2^-10 for their error;
+//   |q - x| >= |q~ - x~| - |q - q~| - |x - x~| >= |q~ - x~| - eq - 2^-8 / (1 - 2^-8)
+//     |x~|, with eq = |q - q~| measured once per query (make_query_bf16) and
+//     the norms from sxx and qq enlarged by 2^-10 for their error;
 //   vec_dist's FP32 sum of D non-negative rounded squares >= |q - x|^2
 //     (1 - (D + 2) u).
 // Each step is taken with slack for the FP32 rounding of the bound itself
@@ -2332,11 +2345,13 @@ __device__ __forceinline__ float screen_bound(const VecRegs<D, kBF16>& r, const 
   sxq = sxq + __shfl_xor(sxq, 1);
   sxx = sxx + __shfl_xor(sxx, 1);
   const float qq = reinterpret_cast<const float*>(qb)[dstride / 2];
-  const float nsum = (sqrtf(fmaxf(sxx, 0.f)) + sqrtf(fmaxf(qq, 0.f))) * (1.0f + 0x1p-10f);
+  const float eq = reinterpret_cast<const float*>(qb)[dstride / 2 + 1];
+  const float nx = sqrtf(fmaxf(sxx, 0.f)) * (1.0f + 0x1p-10f);
+  const float nsum = nx + sqrtf(fmaxf(qq, 0.f)) * (1.0f + 0x1p-10f);
   const float e = (float)(D + 8) * 0x1p-21f * nsum * nsum;
   const float dlo = ((sxx - 2.0f * sxq) + qq) - e;
   const float rr = dlo > 0.f ? sqrtf(dlo) * 0.99999f : 0.f;
-  const float rp = rr - (nsum * (0x1p-8f / (1.0f - 0x1p-8f)) * 1.00001f + 1e-30f);
+  const float rp = rr - ((eq + nx * (0x1p-8f / (1.0f - 0x1p-8f))) * 1.00001f + 1e-30f);
   const float c3 = (float)(D + 8) * 0x1p-24f + 0x1p-20f;
   return rp > 0.f ? (rp * rp) * (1.0f - c3) : (rp == rp ? 0.f : rp);
 }
