@@ -81,3 +81,40 @@ def test_rebuild_replaces_a_loaded_index():
     ids, dists, _ = idx.bruteForce(q[:20], 5)
     oi, od = O.brute_force(x, q[:20], 5, "l2")
     assert F.recall_tie_aware(ids, oi, od, 5, rel=1e-5) == 1.0
+
+
+@pytest.mark.parametrize("dim", [100, 96])
+def test_built_l2_index_screened_coop_equals_beam_model(dim):
+    # the BF16 screen on the specialised dimensions of an l2 index (100 has a
+    # tail of 4 coordinates past the last 16): the screened cooperative search,
+    # with every bound checked on the device, equals the FP32 beam model
+    require_gpu()
+    x = D.gmm(20_000, dim, 43)
+    q = D.gmm(300, dim, 44)
+    idx = nm.init(method="hnsw", space="l2")
+    idx.addDataPointBatch(x)
+    idx.createIndex({"M": 16, "efConstruction": 100, "indexThreadQty": 8})
+    path = os.path.join(_TMP, f"l2_screen_{dim}.hnsw")
+    idx.saveIndex(path)
+    oracle = O.OracleIndex(path)
+    idx.setBeam(64)
+    idx.setBeamWaves(8)
+    idx.setScreen(2)
+    idx.collectStats(True)
+    try:
+        for k, ef in [(10, 64), (10, 400)]:
+            idx.setQueryTimeParams({"efSearch": ef, "algoType": "v1merge"})
+            ids, dists, counts = idx.knnQueryBatchArrays(q, k)
+            st = idx.lastStats()
+            print(f"dim {dim} ef {ef}: screened {st['screened']} passed {st['screen_pass']}",
+                  flush=True)
+            assert 0 < st["screen_pass"] < st["screened"]
+            oi, od, oc = oracle.search_beam(q, k, ef, 64, num_threads=8)
+            np.testing.assert_array_equal(counts, oc)
+            np.testing.assert_array_equal(ids, oi)
+            np.testing.assert_array_equal(dists.view(np.uint32), od.view(np.uint32))
+    finally:
+        idx.collectStats(False)
+        idx.setScreen(0)
+        idx.setBeamWaves(0)
+        idx.setBeam(0)
