@@ -624,7 +624,14 @@ def main():
                  else "synthetic (seeded generator; reference-built index)"),
         "config": {"workload": "1,000,000 x 128 l2 half-normal u8-valued, M=16 efC=500, "
                                "10,000 queries, k=10",
-                   "headline_form": head["form"], "efSearch": head["efSearch"],
+                   "headline_form": head["form"],
+                   "headline_note": ("FP32 distances and results, identical to the form without "
+                                     "the screen; each candidate's BF16 copy gives a proven lower "
+                                     "bound on its FP32 distance, and candidates whose bound "
+                                     "reaches the admission key skip their FP32 vector"
+                                     if form_spec(head["form"])[4] else
+                                     "FP32 distances and results"),
+                   "efSearch": head["efSearch"],
                    "algoType": args.algo, "recall_at_10": head["recall_at_10"],
                    "latency_us_p50": head["latency_us_p50"],
                    "latency_us_p99": head["latency_us_p99"],

@@ -176,9 +176,10 @@ int nmsamd_set_beam(nmsamd_index* h, int beam);
  * second computes each step's distances while the first rebuilds the sorted
  * array from the last merge; at most three queries per CU), 8 (the
  * cooperative kernel: a block of eight waves per query sharing every phase
- * of a step, one query per CU, the low-latency form; also taken by beam
- * widths above 16), or 0 (default) = two where the LDS footprint limits a CU
- * to three queries.  Results do not depend on it.  No reference counterpart. */
+ * of a step, two queries per CU where LDS holds two, else one; also taken by
+ * beam widths above 16), or 0 (default) = two where the LDS footprint limits a
+ * CU to three queries.  Results do not depend on it.  No reference
+ * counterpart. */
 int nmsamd_set_beam_waves(nmsamd_index* h, int waves);
 
 /* Vector loads in flight in the one-wave beam kernels: 1 = the plain kernel

@@ -252,9 +252,10 @@ class FloatIndex:
     def setBeamWaves(self, waves: int = 0):
         """Waves per beam query: 1, 2 (the second wave computes each step's
         distances while the first rebuilds the sorted array from the last
-        merge), 8 (the cooperative kernel: one query per CU, every phase of
-        a step shared by eight waves; the low-latency form) or 0 = chosen
-        from the LDS footprint.  Results do not depend on it."""
+        merge), 8 (the cooperative kernel: every phase of a step shared by a
+        block of eight waves, two queries per CU where LDS holds two; the
+        fastest and the low-latency form at large ef) or 0 = chosen from the
+        LDS footprint.  Results do not depend on it."""
         _check(load_library().nmsamd_set_beam_waves(self._h, int(waves)))
 
     def setBeamGather(self, gather: int = 0):
